@@ -1,0 +1,1016 @@
+// Engine: device arena, per-step launch plan, the on-device sampling chain, and the C ABI (include/gsamp_amd.h).
+//
+// Per step k of a string the host enqueues, on the engine's stream and with no host synchronisation:
+//   [norm sweep of step k]  ->  decide kernel  ->  write sweep of step k
+// The decide kernel (one block) reduces the sweeps' per-block partials in a fixed order (deterministic), finishes the
+// bookkeeping of step k-1's write (chain1 / run_steps conditionals, the stored table's norm) and makes step k's
+// decision on the device: the branch (forced, or U_k < p1 with U_k drawn on the host beforehand -- the reference
+// draws one uniform per sampled step from a stream that does not depend on p1, gsamp2.cpp:47, :86, :108), log2 p, and
+// the scale of the write.  Growth steps (fresh slot, x -> tau) need no norm sweep: both branch norms are
+// (|M_b0|^2 + |M_b1|^2) times the norm of the stored table, which the previous write sweep returns.
+//
+// Modes (gsamp2.cpp): chain (sample, sample2, prob) :39-55; chain1 (chain) :56-70; run_steps (hybrid, prefix,
+// resume) :77-120; dumps :121-125, :162-208.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/gsamp_amd.h"
+#include "gs_device.h"
+#include "gs_plan.h"
+
+namespace gs {
+void* sweep_kernel_for(int storage, int v, int ntq, int route, int sweep);
+int preferred_v(int storage);
+}  // namespace gs
+
+using namespace gs;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                      \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) return fail(17, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------------------------
+// small kernels
+
+struct DecideArgs {
+    int k;          // step to decide; k == k1 finishes the run
+    int k0, k1;     // this run covers steps k0..k1-1
+    int ns;
+    int mode;       // ChainMode
+    int norm;       // -1 none, kGram, kDirect: which partials of step k are in part_n
+    int nparts_n;   // blocks of step k's norm sweep
+    int nparts_w;   // blocks of step k-1's write sweep (0 when k == k0)
+};
+
+// Sum nparts partials (stride kPartStride) into out[0..nq) in a fixed order: thread t sums t, t+256, ... then a
+// fixed tree.  All threads must call it.
+__device__ void reduce_parts(const double* part, int nparts, int nq, double* out, double (*sh)[4]) {
+    const int t = threadIdx.x;
+    double acc[4] = {0, 0, 0, 0};
+    for (int i = t; i < nparts; i += kThreads)
+        for (int q = 0; q < nq; q++) acc[q] += part[size_t(i) * kPartStride + q];
+    for (int q = 0; q < 4; q++) sh[t][q] = acc[q];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (t < w)
+            for (int q = 0; q < 4; q++) sh[t][q] += sh[t + w][q];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < nq; q++) out[q] = sh[0][q];
+    __syncthreads();
+}
+
+__device__ double2 step_m(const KStep& s, int b, int x) { return make_double2(s.mre[b][x], s.mim[b][x]); }
+
+__global__ __launch_bounds__(kThreads) void decide_kernel(const KStep* __restrict__ steps, Ctl* __restrict__ ctl,
+                                                          const double* __restrict__ part_n,
+                                                          const double* __restrict__ part_w, DecideArgs a,
+                                                          const int8_t* __restrict__ forced,
+                                                          const double* __restrict__ U, int8_t* __restrict__ bits,
+                                                          double* __restrict__ cond, double* __restrict__ out) {
+    __shared__ double sh[kThreads][4];
+    __shared__ double red_n[4], red_w[4];
+    reduce_parts(part_n, a.norm >= 0 ? a.nparts_n : 0, 4, red_n, sh);
+    reduce_parts(part_w, a.nparts_w, 2, red_w, sh);
+    if (threadIdx.x != 0) return;
+    Ctl& c = *ctl;
+    // ---- bookkeeping of the write of step k-1
+    if (!c.dead && a.nparts_w > 0) {
+        const int kp = a.k - 1;
+        const double nb = red_w[0];
+        c.nstored = red_w[1];
+        if (a.mode == kChain1) {
+            cond[kp] = nb / (2 * c.prev);
+            c.prev = nb;
+            if (nb <= 0) {
+                for (int q = kp; q < a.ns; q++) cond[q] = 0;
+                c.logp = -INFINITY;
+                c.dead = 1;
+            }
+        } else if (a.mode == kRun) {
+            c.logscale += 0.5 * log2(c.prev);
+            cond[kp] = nb / 2;
+            c.prev = nb;
+            if (nb <= 0) {
+                for (int q = kp; q < a.k1; q++) {
+                    cond[q] = 0;
+                    bits[q] = forced[q] < 0 ? 0 : forced[q];
+                }
+                c.logp = -INFINITY;
+                c.dead = 1;
+            } else {
+                c.logp += log2(cond[kp]);
+            }
+        }
+    }
+    if (a.k == a.k1) {                                      // finish: publish the run's scalars
+        if (a.mode == kChain1 && !c.dead) c.logp = log2(c.prev) - a.ns;
+        out[0] = c.logp;
+        out[1] = c.logscale;
+        out[2] = c.prev;
+        out[3] = c.dead;
+        return;
+    }
+    if (c.dead) return;
+    // ---- decision of step k
+    const KStep& s = steps[a.k];
+    int bv = forced ? forced[a.k] : -1;
+    double sc = 1.0;
+    bool need_p = (a.mode == kChain) || (a.mode == kRun && bv < 0);
+    double n0 = 0, n1 = 0;
+    if (need_p) {
+        if (a.norm == kGram) {
+            const double g00 = red_n[0], g11 = red_n[1], gr = red_n[2], gi = red_n[3];
+            for (int b = 0; b < 2; b++) {
+                const double2 m0 = step_m(s, b, 0), m1 = step_m(s, b, 1);
+                // 2 Re(m0 conj(m1) G01)
+                const double pr = m0.x * m1.x + m0.y * m1.y, pi = m0.y * m1.x - m0.x * m1.y;
+                const double nbv = (m0.x * m0.x + m0.y * m0.y) * g00 + (m1.x * m1.x + m1.y * m1.y) * g11 +
+                                   2 * (pr * gr - pi * gi);
+                (b ? n1 : n0) = nbv;
+            }
+        } else if (a.norm == kDirect) {
+            n0 = red_n[0];
+            n1 = red_n[1];
+        } else {                                              // growth step
+            for (int b = 0; b < 2; b++) {
+                const double2 m0 = step_m(s, b, 0), m1 = step_m(s, b, 1);
+                (b ? n1 : n0) = (m0.x * m0.x + m0.y * m0.y + m1.x * m1.x + m1.y * m1.y) * c.nstored;
+            }
+        }
+    }
+    if (a.mode == kChain) {
+        const double p1 = n1 / (n0 + n1);
+        if (bv < 0) bv = U[a.k] < p1;
+        const double pb = bv ? p1 : 1 - p1;
+        if (pb <= 0) {
+            c.logp = -INFINITY;
+            c.dead = 1;
+            return;
+        }
+        c.logp += log2(pb);
+        sc = 1.0 / sqrt(bv ? n1 : n0);
+    } else if (a.mode == kChain1) {
+        sc = 1.0;
+    } else {
+        sc = 1.0 / sqrt(c.prev);
+        if (bv < 0) bv = U[c.qidx++] < n1 / (n0 + n1);
+    }
+    bits[a.k] = int8_t(bv);
+    c.bv = bv;
+    c.sc = sc;
+    const double2 m0 = step_m(s, bv, 0), m1 = step_m(s, bv, 1);
+    c.w[0] = m0.x * sc;
+    c.w[1] = m0.y * sc;
+    c.w[2] = m1.x * sc;
+    c.w[3] = m1.y * sc;
+}
+
+template <class S>
+__device__ S to_storage(double2 v);
+template <>
+__device__ float2 to_storage<float2>(double2 v) { return make_float2(float(v.x), float(v.y)); }
+template <>
+__device__ double2 to_storage<double2>(double2 v) { return v; }
+
+template <class S>
+__global__ void init_kernel(S* mem, Ctl* ctl) {
+    mem[0] = to_storage<S>(make_double2(1.0, 0.0));
+    Ctl c{};
+    c.prev = 1;
+    c.nstored = 1;
+    *ctl = c;
+}
+
+__global__ void init_ctl_kernel(Ctl* ctl, double prev, double logscale) {
+    Ctl c{};
+    c.prev = prev;
+    c.nstored = prev;
+    c.logscale = logscale;
+    *ctl = c;
+}
+
+__device__ uint64_t base_of(const KStep& s, uint64_t c) {
+    uint64_t a = 0;
+    for (int j = 0; c; j++, c >>= 1)
+        if (c & 1) a ^= s.cols[j];
+    return a;
+}
+
+// Live table of step K in logical order: out[c * ntq + t] = mem[base(c) ^ offs[t]] (offsets sorted ascending).
+template <class S>
+__global__ void gather_kernel(const KStep* __restrict__ sk, const S* __restrict__ mem, const uint64_t* __restrict__ offs,
+                              int ntq, int64_t n, double2* __restrict__ out) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const uint64_t c = uint64_t(i / ntq);
+        const int t = int(i % ntq);
+        const uint64_t a = (sk ? base_of(*sk, c) : 0) ^ offs[t];
+        const S v = mem[a];
+        out[i] = make_double2(double(v.x), double(v.y));
+    }
+}
+
+// acc[c * ntq + t] += w * tab[(c ^ cshift) * ntq + perm[t]]
+__global__ void slice_accumulate_kernel(double2* __restrict__ acc, const double2* __restrict__ tab, double w,
+                                        uint64_t cshift, const int* __restrict__ perm, int ntq, int64_t n) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const uint64_t c = uint64_t(i / ntq);
+        const int t = int(i % ntq);
+        const double2 v = tab[int64_t(c ^ cshift) * ntq + perm[t]];
+        acc[i] = make_double2(fma(w, v.x, acc[i].x), fma(w, v.y, acc[i].y));
+    }
+}
+
+// mem[base(c) ^ offs[t]] = acc[c * ntq + t]; per-block partial of sum |acc|^2 (FP64, before rounding).
+template <class S>
+__global__ __launch_bounds__(kThreads) void scatter_kernel(const KStep* __restrict__ sk, S* __restrict__ mem,
+                                                           const uint64_t* __restrict__ offs, int ntq, int64_t n,
+                                                           const double2* __restrict__ acc, double* __restrict__ part) {
+    double nrm = 0;
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+        const uint64_t c = uint64_t(i / ntq);
+        const int t = int(i % ntq);
+        const uint64_t a = (sk ? base_of(*sk, c) : 0) ^ offs[t];
+        const double2 v = acc[i];
+        mem[a] = to_storage<S>(v);
+        nrm = fma(v.x, v.x, fma(v.y, v.y, nrm));
+    }
+    for (int o = 32; o > 0; o >>= 1) nrm += __shfl_xor(nrm, o, 64);
+    __shared__ double sh[kThreads / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int w = 0; w < kThreads / 64; w++) t += sh[w];
+        part[size_t(blockIdx.x) * kPartStride] = t;
+    }
+}
+
+__global__ void sum_parts_kernel(const double* __restrict__ part, int nparts, double* __restrict__ out) {
+    __shared__ double sh[kThreads][4];
+    __shared__ double r[4];
+    reduce_parts(part, nparts, 1, r, sh);
+    if (threadIdx.x == 0) out[0] = r[0];
+}
+
+int route_and_relabel(const StepInfo& si, KStep& ks, std::string& err) {
+    // canonical routing (see Route): reorder the offsets so that the term routing is one of four fixed patterns
+    const int n = si.ntq;
+    int order[kMaxOffsets];
+    for (int j = 0; j < n; j++) order[j] = j;
+    int route = -1;
+    if (si.m == 1) {
+        route = kRoute1;
+    } else {
+        bool split = true;
+        for (int j = 0; j < n; j++) split &= si.tau0[j] == 0 && si.tau1[j] == 1;
+        if (split) {
+            route = kRouteSplit;
+        } else {
+            std::vector<int> z, o;
+            for (int j = 0; j < n; j++) (si.tau0[j] ? o : z).push_back(j);
+            if (z.size() != o.size()) { err = "unsupported term routing"; return 15; }
+            for (size_t i = 0; i < z.size(); i++) {
+                order[2 * i] = z[i];
+                order[2 * i + 1] = o[i];
+            }
+            bool mixa = true, mixb = true;
+            for (int jj = 0; jj < n; jj++) {
+                const int j = order[jj];
+                mixa &= si.tau1[j] == 1 - (jj & 1);
+                mixb &= si.tau1[j] == (jj & 1);
+            }
+            if (mixa) route = kRouteMixA;
+            else if (mixb) route = kRouteMixB;
+            else { err = "unsupported term routing"; return 15; }
+        }
+    }
+    ks.route = route;
+    ks.parmask = 0;
+    for (int jj = 0; jj < n; jj++) {
+        ks.off[jj] = si.off[order[jj]];
+        if (si.par[order[jj]]) ks.parmask |= 1u << jj;
+    }
+    return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------------------
+// engine
+
+struct gs_plan {
+    Plan p;
+};
+
+struct gs_engine {
+    Plan plan;
+    int storage = GS_C64;
+    int device = 0;
+    size_t esz = 8;                 // bytes per arena entry
+    hipStream_t stream = nullptr;
+    void* arena = nullptr;
+    KStep* dsteps = nullptr;
+    std::vector<KStep> hsteps;
+    struct Launch {
+        void* gram = nullptr;
+        void* direct = nullptr;
+        void* write = nullptr;
+        int grid = 1;
+    };
+    std::vector<Launch> launch;
+    double* part_n = nullptr;
+    double* part_w = nullptr;
+    Ctl* ctl = nullptr;
+    // per-string buffers, kChunk strings
+    static constexpr int kChunk = 64;
+    int8_t* d_forced = nullptr;
+    double* d_U = nullptr;
+    int8_t* d_bits = nullptr;
+    double* d_cond = nullptr;
+    double* d_out = nullptr;        // [kChunk][4]
+    unsigned char* d_amp = nullptr; // [kChunk][16]
+    int8_t* h_forced = nullptr;     // pinned mirrors
+    double* h_U = nullptr;
+    int8_t* h_bits = nullptr;
+    double* h_cond = nullptr;
+    double* h_out = nullptr;
+    unsigned char* h_amp = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0;
+    // optional per-sweep timing (bench / profiling)
+    bool profile = false;
+    std::vector<hipEvent_t> pev;
+    std::vector<int> pev_tag;       // (step << 2) | sweep
+    std::vector<double> prof_ms;    // [ns][3]
+    std::vector<long> prof_n;       // [ns][3]
+};
+
+namespace {
+
+int ns_of(const gs_engine* e) { return e->plan.ns; }
+
+void* arena_void(gs_engine* e) { return e->arena; }
+
+int launch_sweep(gs_engine* e, int k, int sweep) {
+    const auto& L = e->launch[size_t(k)];
+    void* fn = sweep == kGram ? L.gram : sweep == kDirect ? L.direct : L.write;
+    if (!fn) return fail(15, "no kernel for step " + std::to_string(k));
+    const KStep* sp = e->dsteps + k;
+    void* mem = e->arena;
+    double* part = sweep == kWrite ? e->part_w : e->part_n;
+    const Ctl* ctl = e->ctl;
+    void* args[] = {&sp, &mem, &part, &ctl};
+    if (e->profile) {
+        hipEvent_t a, b;
+        HIP_TRY(hipEventCreate(&a));
+        HIP_TRY(hipEventCreate(&b));
+        HIP_TRY(hipEventRecord(a, e->stream));
+        HIP_TRY(hipLaunchKernel(fn, dim3(L.grid), dim3(kThreads), args, 0, e->stream));
+        HIP_TRY(hipEventRecord(b, e->stream));
+        e->pev.push_back(a);
+        e->pev.push_back(b);
+        e->pev_tag.push_back((k << 2) | sweep);
+        return 0;
+    }
+    HIP_TRY(hipLaunchKernel(fn, dim3(L.grid), dim3(kThreads), args, 0, e->stream));
+    return 0;
+}
+
+int collect_profile(gs_engine* e) {
+    for (size_t i = 0; i < e->pev_tag.size(); i++) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e->pev[2 * i], e->pev[2 * i + 1]));
+        const int k = e->pev_tag[i] >> 2, sw = e->pev_tag[i] & 3;
+        e->prof_ms[size_t(k) * 3 + sw] += ms;
+        e->prof_n[size_t(k) * 3 + sw] += 1;
+        (void)hipEventDestroy(e->pev[2 * i]);
+        (void)hipEventDestroy(e->pev[2 * i + 1]);
+    }
+    e->pev.clear();
+    e->pev_tag.clear();
+    return 0;
+}
+
+int launch_decide(gs_engine* e, const DecideArgs& a, const int8_t* forced, const double* U, int8_t* bits, double* cond,
+                  double* out) {
+    const KStep* steps = e->dsteps;
+    Ctl* ctl = e->ctl;
+    const double* pn = e->part_n;
+    const double* pw = e->part_w;
+    DecideArgs aa = a;
+    void* args[] = {&steps, &ctl, &pn, &pw, &aa, &forced, &U, &bits, &cond, &out};
+    HIP_TRY(hipLaunchKernel(reinterpret_cast<void*>(&decide_kernel), dim3(1), dim3(kThreads), args, 0, e->stream));
+    return 0;
+}
+
+int launch_init(gs_engine* e) {
+    Ctl* ctl = e->ctl;
+    if (e->storage == GS_C64) {
+        float2* mem = static_cast<float2*>(e->arena);
+        hipLaunchKernelGGL(init_kernel<float2>, dim3(1), dim3(1), 0, e->stream, mem, ctl);
+    } else {
+        double2* mem = static_cast<double2*>(e->arena);
+        hipLaunchKernelGGL(init_kernel<double2>, dim3(1), dim3(1), 0, e->stream, mem, ctl);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Which norm sweep step k needs in a mode (-1: none).
+int norm_for(const gs_engine* e, int mode, int k, int forced_bit, bool forced_string) {
+    const KStep& s = e->hsteps[size_t(k)];
+    if (mode == kChain1) return -1;
+    if (mode == kRun && forced_bit >= 0) return -1;
+    if (s.growth) return -1;
+    if (mode == kChain && forced_string) return kDirect;
+    return kGram;
+}
+
+// Enqueue steps k0..k1-1 of one string plus the finishing decide.  forced_h: host copy of the string's forced bits
+// (nullptr when every step is sampled); device pointers address this string's slot.
+int enqueue_run(gs_engine* e, int mode, int k0, int k1, const int8_t* forced_h, const int8_t* forced_d, const double* U_d,
+                int8_t* bits_d, double* cond_d, double* out_d) {
+    int prev_grid = 0;
+    for (int k = k0; k <= k1; k++) {
+        DecideArgs a{};
+        a.k = k;
+        a.k0 = k0;
+        a.k1 = k1;
+        a.ns = e->plan.ns;
+        a.mode = mode;
+        a.norm = -1;
+        a.nparts_w = prev_grid;
+        if (k < k1) {
+            const int fb = forced_h ? forced_h[k] : -1;
+            a.norm = norm_for(e, mode, k, fb, forced_h != nullptr && mode == kChain);
+            if (a.norm >= 0) {
+                if (int rc = launch_sweep(e, k, a.norm)) return rc;
+                a.nparts_n = e->launch[size_t(k)].grid;
+            }
+        }
+        if (int rc = launch_decide(e, a, forced_d, U_d, bits_d, cond_d, out_d)) return rc;
+        if (k < k1) {
+            if (int rc = launch_sweep(e, k, kWrite)) return rc;
+            prev_grid = e->launch[size_t(k)].grid;
+        }
+    }
+    return 0;
+}
+
+int build_steps(gs_engine* e) {
+    const Plan& P = e->plan;
+    e->hsteps.assign(size_t(P.ns), KStep{});
+    e->launch.assign(size_t(P.ns), gs_engine::Launch{});
+    const int vmax = preferred_v(e->storage);
+    for (int k = 0; k < P.ns; k++) {
+        const StepInfo& si = P.steps[size_t(k)];
+        KStep& ks = e->hsteps[size_t(k)];
+        memset(&ks, 0, sizeof ks);
+        for (int j = 0; j < si.nk; j++) ks.cols[j] = si.cols[size_t(j)];
+        ks.g = si.g;
+        ks.rhoc = si.rhoc;
+        for (int b = 0; b < 2; b++)
+            for (int x = 0; x < 2; x++) {
+                ks.mre[b][x] = si.M[b][x].real();
+                ks.mim[b][x] = si.M[b][x].imag();
+            }
+        ks.nk = si.nk;
+        ks.ntq = si.ntq;
+        ks.m = si.m;
+        ks.growth = si.growth ? 1 : 0;
+        std::string err;
+        if (int rc = route_and_relabel(si, ks, err)) return fail(rc, err + " at step " + std::to_string(k));
+        // cosets per thread: 2^vmax when the low vmax kept columns are the unit bits 0..vmax-1
+        int unit = 0;
+        while (unit < si.nk && unit < vmax && si.cols[size_t(unit)] == (uint64_t(1) << unit)) unit++;
+        ks.v = unit == vmax ? vmax : 0;
+        const int lt = ks.v + 8;
+        if (si.nk <= lt) {
+            ks.lg_grid = 0;
+            ks.lg_iter = 0;
+        } else {
+            const int tiles = si.nk - lt;
+            ks.lg_grid = std::min(tiles, kMaxGridLog2);
+            ks.lg_iter = tiles - ks.lg_grid;
+        }
+        auto& L = e->launch[size_t(k)];
+        L.grid = 1 << ks.lg_grid;
+        L.gram = sweep_kernel_for(e->storage, ks.v, ks.ntq, ks.route, kGram);
+        L.direct = sweep_kernel_for(e->storage, ks.v, ks.ntq, ks.route, kDirect);
+        L.write = sweep_kernel_for(e->storage, ks.v, ks.ntq, ks.route, kWrite);
+        if (!L.gram || !L.direct || !L.write) return fail(15, "no kernel for step " + std::to_string(k));
+    }
+    return 0;
+}
+
+void free_engine(gs_engine* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    (void)hipFree(e->arena);
+    (void)hipFree(e->dsteps);
+    (void)hipFree(e->part_n);
+    (void)hipFree(e->part_w);
+    (void)hipFree(e->ctl);
+    (void)hipFree(e->d_forced);
+    (void)hipFree(e->d_U);
+    (void)hipFree(e->d_bits);
+    (void)hipFree(e->d_cond);
+    (void)hipFree(e->d_out);
+    (void)hipFree(e->d_amp);
+    (void)hipHostFree(e->h_forced);
+    (void)hipHostFree(e->h_U);
+    (void)hipHostFree(e->h_bits);
+    (void)hipHostFree(e->h_cond);
+    (void)hipHostFree(e->h_out);
+    (void)hipHostFree(e->h_amp);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int begin_timing(gs_engine* e) {
+    HIP_TRY(hipEventRecord(e->ev0, e->stream));
+    return 0;
+}
+
+int end_timing(gs_engine* e) {
+    HIP_TRY(hipEventRecord(e->ev1, e->stream));
+    HIP_TRY(hipEventSynchronize(e->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    e->last_ms += ms;
+    if (e->profile) return collect_profile(e);
+    return 0;
+}
+
+std::complex<double> read_amp(const gs_engine* e, int slot) {
+    const unsigned char* p = e->h_amp + size_t(slot) * 16;
+    if (e->storage == GS_C64) {
+        float v[2];
+        memcpy(v, p, 8);
+        return {double(v[0]), double(v[1])};
+    }
+    double v[2];
+    memcpy(v, p, 16);
+    return {v[0], v[1]};
+}
+
+// Copy arena entry 0 into the string's amplitude slot (ordered on the stream).
+int enqueue_amp(gs_engine* e, int slot) {
+    HIP_TRY(hipMemcpyAsync(e->d_amp + size_t(slot) * 16, e->arena, e->esz, hipMemcpyDeviceToDevice, e->stream));
+    return 0;
+}
+
+int check_bits(const int8_t* b, int64_t n, bool allow_q) {
+    for (int64_t i = 0; i < n; i++)
+        if (!(b[i] == 0 || b[i] == 1 || (allow_q && b[i] == -1)))
+            return fail(16, "bit values must be 0 or 1" + std::string(allow_q ? " or -1" : ""));
+    return 0;
+}
+
+// Forced-string modes (prob, chain): strings are processed in chunks of kChunk.
+int run_forced(gs_engine* e, int mode, const int8_t* bits, int64_t nstr, double* log2p, double* cond, double* amp) {
+    const int ns = ns_of(e);
+    if (int rc = check_bits(bits, nstr * ns, false)) return rc;
+    HIP_TRY(hipSetDevice(e->device));
+    e->last_ms = 0;
+    for (int64_t s0 = 0; s0 < nstr; s0 += gs_engine::kChunk) {
+        const int cnt = int(std::min<int64_t>(gs_engine::kChunk, nstr - s0));
+        memcpy(e->h_forced, bits + s0 * ns, size_t(cnt) * ns);
+        HIP_TRY(hipMemcpyAsync(e->d_forced, e->h_forced, size_t(cnt) * ns, hipMemcpyHostToDevice, e->stream));
+        if (int rc = begin_timing(e)) return rc;
+        for (int i = 0; i < cnt; i++) {
+            if (int rc = launch_init(e)) return rc;
+            if (int rc = enqueue_run(e, mode, 0, ns, e->h_forced + size_t(i) * ns, e->d_forced + size_t(i) * ns, e->d_U,
+                                     e->d_bits + size_t(i) * ns, e->d_cond + size_t(i) * ns, e->d_out + size_t(i) * 4))
+                return rc;
+            if (amp)
+                if (int rc = enqueue_amp(e, i)) return rc;
+        }
+        if (int rc = end_timing(e)) return rc;
+        HIP_TRY(hipMemcpyAsync(e->h_out, e->d_out, size_t(cnt) * 4 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        if (cond)
+            HIP_TRY(hipMemcpyAsync(e->h_cond, e->d_cond, size_t(cnt) * ns * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        if (amp) HIP_TRY(hipMemcpyAsync(e->h_amp, e->d_amp, size_t(cnt) * 16, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        for (int i = 0; i < cnt; i++) {
+            log2p[s0 + i] = e->h_out[i * 4];
+            if (cond) memcpy(cond + (s0 + i) * ns, e->h_cond + size_t(i) * ns, size_t(ns) * sizeof(double));
+            if (amp) {
+                const std::complex<double> z = read_amp(e, i);
+                amp[2 * (s0 + i)] = z.real();
+                amp[2 * (s0 + i) + 1] = z.imag();
+            }
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char* gs_last_error(void) { return g_err.c_str(); }
+
+int gs_plan_load(const char* path, gs_plan** out) {
+    if (!path || !out) return fail(16, "null argument");
+    auto* p = new gs_plan;
+    std::string err;
+    if (int rc = parse_plan_file(path, p->p, err)) {
+        delete p;
+        return fail(rc, err);
+    }
+    *out = p;
+    return 0;
+}
+
+int gs_plan_parse(const char* text, gs_plan** out) {
+    if (!text || !out) return fail(16, "null argument");
+    auto* p = new gs_plan;
+    std::string err;
+    if (int rc = parse_plan_text(text, p->p, err)) {
+        delete p;
+        return fail(rc, err);
+    }
+    *out = p;
+    return 0;
+}
+
+int gs_plan_info(const gs_plan* plan, int* n, int* ns, int* R) {
+    if (!plan) return fail(16, "null plan");
+    if (n) *n = plan->p.n;
+    if (ns) *ns = plan->p.ns;
+    if (R) *R = plan->p.R;
+    return 0;
+}
+
+void gs_plan_destroy(gs_plan* plan) { delete plan; }
+
+int gs_engine_create(const gs_plan* plan, int storage, int device, gs_engine** out) {
+    if (!plan || !out) return fail(16, "null argument");
+    if (storage != GS_C64 && storage != GS_C128) return fail(16, "storage must be GS_C64 or GS_C128");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(17, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(16, "device index out of range");
+    auto* e = new gs_engine;
+    e->plan = plan->p;
+    e->storage = storage;
+    e->device = device;
+    e->esz = storage == GS_C64 ? 8 : 16;
+    auto bail = [&](int rc) {
+        free_engine(e);
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bail(fail(17, "hipSetDevice failed"));
+    if (int rc = build_steps(e)) return bail(rc);
+    const int ns = e->plan.ns;
+    const size_t bytes = (size_t(1) << e->plan.R) * e->esz;
+    if (hipMalloc(&e->arena, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        e->arena = nullptr;
+        return bail(fail(11, "allocation of " + std::to_string(bytes) + " bytes failed"));
+    }
+    auto dev_alloc = [&](void** p, size_t n) { return hipMalloc(p, std::max<size_t>(n, 16)) == hipSuccess; };
+    auto host_alloc = [&](void** p, size_t n) { return hipHostMalloc(p, std::max<size_t>(n, 16), 0) == hipSuccess; };
+    const size_t C = gs_engine::kChunk;
+    const size_t nsz = size_t(std::max(ns, 1));
+    bool ok = dev_alloc((void**)&e->dsteps, sizeof(KStep) * nsz) &&
+              dev_alloc((void**)&e->part_n, sizeof(double) * kPartStride << kMaxGridLog2) &&
+              dev_alloc((void**)&e->part_w, sizeof(double) * kPartStride << kMaxGridLog2) &&
+              dev_alloc((void**)&e->ctl, sizeof(Ctl)) && dev_alloc((void**)&e->d_forced, C * nsz) &&
+              dev_alloc((void**)&e->d_U, sizeof(double) * C * nsz) && dev_alloc((void**)&e->d_bits, C * nsz) &&
+              dev_alloc((void**)&e->d_cond, sizeof(double) * C * nsz) &&
+              dev_alloc((void**)&e->d_out, sizeof(double) * C * 4) && dev_alloc((void**)&e->d_amp, 16 * C) &&
+              host_alloc((void**)&e->h_forced, C * nsz) && host_alloc((void**)&e->h_U, sizeof(double) * C * nsz) &&
+              host_alloc((void**)&e->h_bits, C * nsz) && host_alloc((void**)&e->h_cond, sizeof(double) * C * nsz) &&
+              host_alloc((void**)&e->h_out, sizeof(double) * C * 4) && host_alloc((void**)&e->h_amp, 16 * C);
+    if (!ok) return bail(fail(11, "allocation of engine buffers failed"));
+    if (hipStreamCreate(&e->stream) != hipSuccess || hipEventCreate(&e->ev0) != hipSuccess ||
+        hipEventCreate(&e->ev1) != hipSuccess)
+        return bail(fail(17, "stream/event creation failed"));
+    if (ns > 0 && hipMemcpy(e->dsteps, e->hsteps.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(17, "plan upload failed"));
+    e->prof_ms.assign(nsz * 3, 0.0);
+    e->prof_n.assign(nsz * 3, 0);
+    *out = e;
+    return 0;
+}
+
+void gs_engine_destroy(gs_engine* eng) { free_engine(eng); }
+
+int gs_prob(gs_engine* eng, const int8_t* bits, int64_t nstr, double* log2p) {
+    if (!eng || (nstr > 0 && (!bits || !log2p)) || nstr < 0) return fail(16, "bad argument");
+    return run_forced(eng, kChain, bits, nstr, log2p, nullptr, nullptr);
+}
+
+int gs_chain(gs_engine* eng, const int8_t* bits, int64_t nstr, double* log2p, double* cond, double* amp_reim) {
+    if (!eng || (nstr > 0 && (!bits || !log2p || !cond)) || nstr < 0) return fail(16, "bad argument");
+    return run_forced(eng, kChain1, bits, nstr, log2p, cond, amp_reim);
+}
+
+int gs_sample(gs_engine* eng, uint64_t seed, int64_t first, int64_t count, int seeding, int8_t* bits, double* log2p) {
+    if (!eng || count < 0 || first < 0 || (count > 0 && (!bits || !log2p))) return fail(16, "bad argument");
+    if (seeding != GS_SEED_PLAIN && seeding != GS_SEED_SEQ) return fail(16, "bad seeding");
+    gs_engine* e = eng;
+    const int ns = ns_of(e);
+    HIP_TRY(hipSetDevice(e->device));
+    e->last_ms = 0;
+    for (int64_t s0 = 0; s0 < count; s0 += gs_engine::kChunk) {
+        const int cnt = int(std::min<int64_t>(gs_engine::kChunk, count - s0));
+        for (int i = 0; i < cnt; i++) {
+            const unsigned long long idx = (unsigned long long)(first + s0 + i);
+            std::mt19937_64 rng;
+            if (seeding == GS_SEED_PLAIN) {
+                rng.seed(seed + idx);
+            } else {
+                std::seed_seq sq{unsigned(seed & 0xffffffffu), unsigned(seed >> 32), unsigned(idx & 0xffffffffu),
+                                 unsigned(idx >> 32), 0x9E3779B9u};
+                rng.seed(sq);
+            }
+            std::uniform_real_distribution<double> U(0, 1);
+            for (int k = 0; k < ns; k++) e->h_U[size_t(i) * ns + k] = U(rng);
+        }
+        HIP_TRY(hipMemcpyAsync(e->d_U, e->h_U, sizeof(double) * size_t(cnt) * ns, hipMemcpyHostToDevice, e->stream));
+        if (int rc = begin_timing(e)) return rc;
+        for (int i = 0; i < cnt; i++) {
+            if (int rc = launch_init(e)) return rc;
+            if (int rc = enqueue_run(e, kChain, 0, ns, nullptr, nullptr, e->d_U + size_t(i) * ns,
+                                     e->d_bits + size_t(i) * ns, e->d_cond + size_t(i) * ns, e->d_out + size_t(i) * 4))
+                return rc;
+        }
+        if (int rc = end_timing(e)) return rc;
+        HIP_TRY(hipMemcpyAsync(e->h_bits, e->d_bits, size_t(cnt) * ns, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->h_out, e->d_out, size_t(cnt) * 4 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        memcpy(bits + s0 * ns, e->h_bits, size_t(cnt) * ns);
+        for (int i = 0; i < cnt; i++) log2p[s0 + i] = e->h_out[i * 4];
+    }
+    return 0;
+}
+
+int gs_hybrid(gs_engine* eng, const int8_t* bits, int64_t nstr, uint64_t seed, int64_t first_line, int8_t* out_bits,
+              double* log2p, double* cond) {
+    if (!eng || nstr < 0 || (nstr > 0 && (!bits || !out_bits || !log2p || !cond))) return fail(16, "bad argument");
+    gs_engine* e = eng;
+    const int ns = ns_of(e);
+    if (int rc = check_bits(bits, nstr * ns, true)) return rc;
+    HIP_TRY(hipSetDevice(e->device));
+    e->last_ms = 0;
+    for (int64_t s0 = 0; s0 < nstr; s0 += gs_engine::kChunk) {
+        const int cnt = int(std::min<int64_t>(gs_engine::kChunk, nstr - s0));
+        memcpy(e->h_forced, bits + s0 * ns, size_t(cnt) * ns);
+        for (int i = 0; i < cnt; i++) {
+            std::mt19937_64 rng(seed + (unsigned long long)(first_line + s0 + i));
+            std::uniform_real_distribution<double> U(0, 1);
+            int q = 0;
+            for (int k = 0; k < ns; k++)
+                if (e->h_forced[size_t(i) * ns + k] < 0) e->h_U[size_t(i) * ns + q++] = U(rng);
+        }
+        HIP_TRY(hipMemcpyAsync(e->d_forced, e->h_forced, size_t(cnt) * ns, hipMemcpyHostToDevice, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->d_U, e->h_U, sizeof(double) * size_t(cnt) * ns, hipMemcpyHostToDevice, e->stream));
+        if (int rc = begin_timing(e)) return rc;
+        for (int i = 0; i < cnt; i++) {
+            if (int rc = launch_init(e)) return rc;
+            if (int rc = enqueue_run(e, kRun, 0, ns, e->h_forced + size_t(i) * ns, e->d_forced + size_t(i) * ns,
+                                     e->d_U + size_t(i) * ns, e->d_bits + size_t(i) * ns, e->d_cond + size_t(i) * ns,
+                                     e->d_out + size_t(i) * 4))
+                return rc;
+        }
+        if (int rc = end_timing(e)) return rc;
+        HIP_TRY(hipMemcpyAsync(e->h_bits, e->d_bits, size_t(cnt) * ns, hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->h_cond, e->d_cond, size_t(cnt) * ns * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipMemcpyAsync(e->h_out, e->d_out, size_t(cnt) * 4 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        memcpy(out_bits + s0 * ns, e->h_bits, size_t(cnt) * ns);
+        memcpy(cond + s0 * ns, e->h_cond, size_t(cnt) * ns * sizeof(double));
+        for (int i = 0; i < cnt; i++) log2p[s0 + i] = e->h_out[i * 4];
+    }
+    return 0;
+}
+
+int gs_prefix(gs_engine* eng, const int8_t* bits, int K, double* table_reim, int64_t table_cap, int* nk, int* ntq,
+              double* logscale, double* log2p_prefix) {
+    if (!eng || !bits || !nk || !ntq) return fail(16, "bad argument");
+    gs_engine* e = eng;
+    const int ns = ns_of(e);
+    if (K < 0 || K > ns) return fail(16, "K out of range");
+    for (int k = 0; k < K; k++)
+        if (bits[k] == -1) return fail(8, "'?' before K in prefix mode");
+    if (int rc = check_bits(bits, K, false)) return rc;
+    std::vector<uint64_t> offs = K < ns ? e->plan.steps[size_t(K)].sorted_offsets() : std::vector<uint64_t>{0};
+    *nk = K < ns ? e->plan.steps[size_t(K)].nk : 0;
+    *ntq = int(offs.size());
+    if (!table_reim) return 0;
+    const int64_t n = (int64_t(1) << *nk) * *ntq;
+    if (table_cap < n) return fail(16, "table capacity too small");
+    HIP_TRY(hipSetDevice(e->device));
+    e->last_ms = 0;
+    memcpy(e->h_forced, bits, size_t(ns));
+    for (int k = K; k < ns; k++) e->h_forced[k] = 0;
+    HIP_TRY(hipMemcpyAsync(e->d_forced, e->h_forced, size_t(ns), hipMemcpyHostToDevice, e->stream));
+    if (int rc = begin_timing(e)) return rc;
+    if (int rc = launch_init(e)) return rc;
+    if (int rc = enqueue_run(e, kRun, 0, K, e->h_forced, e->d_forced, e->d_U, e->d_bits, e->d_cond, e->d_out)) return rc;
+    if (int rc = end_timing(e)) return rc;
+    void* doffs = nullptr;
+    void* dtab = nullptr;
+    HIP_TRY(hipMalloc(&doffs, sizeof(uint64_t) * offs.size()));
+    if (hipMalloc(&dtab, sizeof(double2) * size_t(n)) != hipSuccess) {
+        (void)hipFree(doffs);
+        return fail(11, "allocation of the dump buffer failed");
+    }
+    HIP_TRY(hipMemcpy(doffs, offs.data(), sizeof(uint64_t) * offs.size(), hipMemcpyHostToDevice));
+    const KStep* sk = K < ns ? e->dsteps + K : nullptr;
+    const int grid = int(std::min<int64_t>((n + kThreads - 1) / kThreads, 65536));
+    if (e->storage == GS_C64)
+        hipLaunchKernelGGL(gather_kernel<float2>, dim3(grid), dim3(kThreads), 0, e->stream, sk,
+                           static_cast<const float2*>(e->arena), static_cast<const uint64_t*>(doffs), *ntq, n,
+                           static_cast<double2*>(dtab));
+    else
+        hipLaunchKernelGGL(gather_kernel<double2>, dim3(grid), dim3(kThreads), 0, e->stream, sk,
+                           static_cast<const double2*>(e->arena), static_cast<const uint64_t*>(doffs), *ntq, n,
+                           static_cast<double2*>(dtab));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(e->h_out, e->d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipMemcpyAsync(table_reim, dtab, sizeof(double2) * size_t(n), hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(hipStreamSynchronize(e->stream));
+    (void)hipFree(doffs);
+    (void)hipFree(dtab);
+    if (logscale) *logscale = e->h_out[1];
+    if (log2p_prefix) *log2p_prefix = e->h_out[0];
+    return 0;
+}
+
+int gs_resume(gs_engine* eng, const int8_t* bits, int K, int nslices, const double* const* tables,
+              const double* logscales, const uint64_t* cshift, const uint64_t* tqshift, uint64_t seed, int64_t line,
+              int8_t* out_bits, double* log2p_full, double* log2p_prefix, double* cond) {
+    if (!eng || !bits || nslices < 1 || !tables || !logscales || !out_bits || !log2p_full || !log2p_prefix || !cond)
+        return fail(16, "bad argument");
+    gs_engine* e = eng;
+    const int ns = ns_of(e);
+    if (K < 0 || K > ns) return fail(16, "K out of range");
+    if (int rc = check_bits(bits, ns, true)) return rc;
+    const int nk = K < ns ? e->plan.steps[size_t(K)].nk : 0;
+    std::vector<uint64_t> offs = K < ns ? e->plan.steps[size_t(K)].sorted_offsets() : std::vector<uint64_t>{0};
+    const int ntq = int(offs.size());
+    const int64_t n = (int64_t(1) << nk) * ntq;
+    // twist: per slice, a shift of the kept-coordinate index and a permutation of the sorted offsets
+    std::vector<int> perms(size_t(nslices) * ntq);
+    for (int s = 0; s < nslices; s++) {
+        if (cshift && (nk < 64 ? (cshift[s] >> nk) != 0 : false))
+            return fail(14, "twist cshift exceeds the kept coordinates");
+        for (int t = 0; t < ntq; t++) {
+            const uint64_t want = offs[size_t(t)] ^ (tqshift ? tqshift[s] : 0);
+            auto it = std::find(offs.begin(), offs.end(), want);
+            if (it == offs.end()) return fail(14, "twist tqshift does not map the coset offsets onto themselves");
+            perms[size_t(s) * ntq + t] = int(it - offs.begin());
+        }
+    }
+    double L = -1e300;
+    for (int s = 0; s < nslices; s++) L = std::max(L, logscales[s]);
+    HIP_TRY(hipSetDevice(e->device));
+    e->last_ms = 0;
+    void *dacc = nullptr, *dtab = nullptr, *doffs = nullptr, *dperm = nullptr;
+    auto cleanup = [&]() {
+        (void)hipFree(dacc);
+        (void)hipFree(dtab);
+        (void)hipFree(doffs);
+        (void)hipFree(dperm);
+    };
+    if (hipMalloc(&dacc, sizeof(double2) * size_t(n)) != hipSuccess ||
+        hipMalloc(&dtab, sizeof(double2) * size_t(n)) != hipSuccess ||
+        hipMalloc(&doffs, sizeof(uint64_t) * offs.size()) != hipSuccess ||
+        hipMalloc(&dperm, sizeof(int) * perms.size()) != hipSuccess) {
+        cleanup();
+        return fail(11, "allocation of the resume buffers failed");
+    }
+    int rc = 0;
+    const int grid = int(std::min<int64_t>((n + kThreads - 1) / kThreads, 2048));
+    do {
+        if (hipMemsetAsync(dacc, 0, sizeof(double2) * size_t(n), e->stream) != hipSuccess ||
+            hipMemcpy(doffs, offs.data(), sizeof(uint64_t) * offs.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dperm, perms.data(), sizeof(int) * perms.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            rc = fail(17, "resume upload failed");
+            break;
+        }
+        for (int s = 0; s < nslices && !rc; s++) {
+            if (hipMemcpyAsync(dtab, tables[s], sizeof(double2) * size_t(n), hipMemcpyHostToDevice, e->stream) != hipSuccess) {
+                rc = fail(17, "resume upload failed");
+                break;
+            }
+            const double w = std::exp2(logscales[s] - L);
+            hipLaunchKernelGGL(slice_accumulate_kernel, dim3(grid), dim3(kThreads), 0, e->stream,
+                               static_cast<double2*>(dacc), static_cast<const double2*>(dtab), w,
+                               cshift ? cshift[s] : 0ull, static_cast<const int*>(dperm) + size_t(s) * ntq, ntq, n);
+        }
+        if (rc) break;
+        const KStep* sk = K < ns ? e->dsteps + K : nullptr;
+        if (e->storage == GS_C64)
+            hipLaunchKernelGGL(scatter_kernel<float2>, dim3(grid), dim3(kThreads), 0, e->stream, sk,
+                               static_cast<float2*>(e->arena), static_cast<const uint64_t*>(doffs), ntq, n,
+                               static_cast<const double2*>(dacc), e->part_w);
+        else
+            hipLaunchKernelGGL(scatter_kernel<double2>, dim3(grid), dim3(kThreads), 0, e->stream, sk,
+                               static_cast<double2*>(e->arena), static_cast<const uint64_t*>(doffs), ntq, n,
+                               static_cast<const double2*>(dacc), e->part_w);
+        hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(kThreads), 0, e->stream, e->part_w, grid, e->d_out + 4);
+        double norm = 0;
+        if (hipMemcpyAsync(&e->h_out[4], e->d_out + 4, sizeof(double), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+            hipStreamSynchronize(e->stream) != hipSuccess) {
+            rc = fail(17, "resume reduction failed");
+            break;
+        }
+        norm = e->h_out[4];
+        const double lp_prefix = -K + std::log2(norm) + 2 * L;
+        // forced bits and uniforms for steps K..ns-1
+        memcpy(e->h_forced, bits, size_t(ns));
+        std::mt19937_64 rng(seed + (unsigned long long)line);
+        std::uniform_real_distribution<double> U(0, 1);
+        int q = 0;
+        for (int k = K; k < ns; k++)
+            if (e->h_forced[k] < 0) e->h_U[q++] = U(rng);
+        if (hipMemcpyAsync(e->d_forced, e->h_forced, size_t(ns), hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+            hipMemcpyAsync(e->d_U, e->h_U, sizeof(double) * size_t(std::max(q, 1)), hipMemcpyHostToDevice, e->stream) != hipSuccess) {
+            rc = fail(17, "resume upload failed");
+            break;
+        }
+        hipLaunchKernelGGL(init_ctl_kernel, dim3(1), dim3(1), 0, e->stream, e->ctl, norm, L);
+        if ((rc = begin_timing(e))) break;
+        if ((rc = enqueue_run(e, kRun, K, ns, e->h_forced, e->d_forced, e->d_U, e->d_bits, e->d_cond, e->d_out))) break;
+        if ((rc = end_timing(e))) break;
+        if (hipMemcpyAsync(e->h_bits, e->d_bits, size_t(ns), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+            hipMemcpyAsync(e->h_cond, e->d_cond, size_t(ns) * sizeof(double), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+            hipMemcpyAsync(e->h_out, e->d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
+            hipStreamSynchronize(e->stream) != hipSuccess) {
+            rc = fail(17, "resume readback failed");
+            break;
+        }
+        for (int k = 0; k < ns; k++) out_bits[k] = k < K ? bits[k] : e->h_bits[k];
+        for (int k = K; k < ns; k++) cond[k] = e->h_cond[k];
+        *log2p_prefix = lp_prefix;
+        *log2p_full = lp_prefix + e->h_out[0];
+    } while (false);
+    cleanup();
+    return rc;
+}
+
+double gs_last_device_ms(const gs_engine* eng) { return eng ? eng->last_ms : 0.0; }
+
+// Profiling hooks (not part of the reference's interface): per-step, per-sweep kernel time from HIP events recorded
+// around every sweep launch on the engine's stream.
+int gs_profile_enable(gs_engine* eng, int on) {
+    if (!eng) return fail(16, "null engine");
+    eng->profile = on != 0;
+    std::fill(eng->prof_ms.begin(), eng->prof_ms.end(), 0.0);
+    std::fill(eng->prof_n.begin(), eng->prof_n.end(), 0);
+    return 0;
+}
+
+int gs_profile_read(const gs_engine* eng, double* ms, int64_t* launches) {
+    if (!eng) return fail(16, "null engine");
+    for (size_t i = 0; i < eng->prof_ms.size(); i++) {
+        if (ms) ms[i] = eng->prof_ms[i];
+        if (launches) launches[i] = eng->prof_n[i];
+    }
+    return 0;
+}
+
+// Launch geometry of a step: cosets per thread (log2), blocks, tiles per block (log2), offsets, route.
+int gs_step_geometry(const gs_engine* eng, int k, int* v, int* grid, int* lg_iter, int* ntq, int* route) {
+    if (!eng || k < 0 || k >= eng->plan.ns) return fail(16, "bad step");
+    const KStep& s = eng->hsteps[size_t(k)];
+    if (v) *v = s.v;
+    if (grid) *grid = 1 << s.lg_grid;
+    if (lg_iter) *lg_iter = s.lg_iter;
+    if (ntq) *ntq = s.ntq;
+    if (route) *route = s.route;
+    return 0;
+}
+
+}  // extern "C"

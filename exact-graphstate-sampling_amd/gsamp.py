@@ -1,0 +1,229 @@
+"""Python front end of the HIP engine (``bin/libgsamp_amd.so``, C ABI in include/gsamp_amd.h).
+
+Mirrors the modes of the reference engine ``gsamp2`` (gsamp2.cpp:149-225) as methods of :class:`Engine`; bit
+strings are numpy int8 arrays in STEP order ([nstr, ns]; -1 = '?').  There is no CPU fallback: constructing an
+engine without the built library or without a HIP device raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "bin", "libgsamp_amd.so")
+CLI_PATH = os.path.join(HERE, "bin", "gsamp2_amd")
+
+C64, C128 = 0, 1
+SEED_PLAIN, SEED_SEQ = 0, 1
+
+# every symbol include/gsamp_amd.h declares
+ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy", "gs_engine_create",
+               "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
+               "gs_last_device_ms", "gs_last_error")
+
+_lib = None
+
+
+class GsError(RuntimeError):
+    """A failed engine call; ``code`` is the reference's exit code for the same failure (see gsamp_amd.h)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def library():
+    """Load the engine library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or build.py) first")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, I64, D, U64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    i8p = ctypes.POINTER(ctypes.c_int8)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "gs_plan_load": (I, [ctypes.c_char_p, pp]),
+        "gs_plan_parse": (I, [ctypes.c_char_p, pp]),
+        "gs_plan_info": (I, [P, ip, ip, ip]),
+        "gs_plan_destroy": (None, [P]),
+        "gs_engine_create": (I, [P, I, I, pp]),
+        "gs_engine_destroy": (None, [P]),
+        "gs_prob": (I, [P, i8p, I64, dp]),
+        "gs_chain": (I, [P, i8p, I64, dp, dp, dp]),
+        "gs_sample": (I, [P, U64, I64, I64, I, i8p, dp]),
+        "gs_hybrid": (I, [P, i8p, I64, U64, I64, i8p, dp, dp]),
+        "gs_prefix": (I, [P, i8p, I, dp, I64, ip, ip, dp, dp]),
+        "gs_resume": (I, [P, i8p, I, I, ctypes.POINTER(dp), dp, ctypes.POINTER(U64), ctypes.POINTER(U64), U64, I64,
+                          i8p, dp, dp, dp]),
+        "gs_last_device_ms": (D, [P]),
+        "gs_last_error": (ctypes.c_char_p, []),
+        "gs_profile_enable": (I, [P, I]),
+        "gs_profile_read": (I, [P, dp, ctypes.POINTER(I64)]),
+        "gs_step_geometry": (I, [P, I, ip, ip, ip, ip, ip]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc:
+        raise GsError(rc, library().gs_last_error().decode())
+
+
+def _i8(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))
+
+
+def _f8(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Plan:
+    """An in-place plan loaded by the engine's parser (same format and error codes as gsamp2.cpp:126-139)."""
+
+    def __init__(self, path=None, text=None):
+        lib = library()
+        self._h = ctypes.c_void_p()
+        if (path is None) == (text is None):
+            raise ValueError("give exactly one of path, text")
+        if path is not None:
+            _check(lib.gs_plan_load(os.fsencode(path), ctypes.byref(self._h)))
+        else:
+            _check(lib.gs_plan_parse(text.encode(), ctypes.byref(self._h)))
+        n, ns, R = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib.gs_plan_info(self._h, ctypes.byref(n), ctypes.byref(ns), ctypes.byref(R))
+        self.n, self.ns, self.R = n.value, ns.value, R.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.gs_plan_destroy(self._h)
+            self._h = None
+
+
+class Engine:
+    """A device arena of 2^R entries (``storage`` C64 or C128) bound to one plan on one HIP device."""
+
+    def __init__(self, plan, storage=C64, device=0):
+        lib = library()
+        self.plan = plan if isinstance(plan, Plan) else Plan(path=plan)
+        self.ns = self.plan.ns
+        self.storage = storage
+        self._h = ctypes.c_void_p()
+        _check(lib.gs_engine_create(self.plan._h, storage, device, ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.gs_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _bits(self, bits, allow_q=False):
+        b = np.ascontiguousarray(np.atleast_2d(np.asarray(bits, dtype=np.int8)))
+        if b.shape[1] != self.ns:
+            raise ValueError(f"strings must have {self.ns} steps, got {b.shape[1]}")
+        return b
+
+    @property
+    def last_device_ms(self):
+        return library().gs_last_device_ms(self._h)
+
+    def prob(self, bits):
+        """log2 Pr of forced strings (``gsamp2 prob``)."""
+        b = self._bits(bits)
+        out = np.empty(b.shape[0])
+        _check(library().gs_prob(self._h, _i8(b), b.shape[0], _f8(out)))
+        return out
+
+    def chain(self, bits):
+        """``(log2 Pr, conditionals [nstr, ns], Z [nstr] complex)`` of forced strings (``gsamp2 chain``)."""
+        b = self._bits(bits)
+        lp = np.empty(b.shape[0])
+        cond = np.empty((b.shape[0], self.ns))
+        amp = np.empty((b.shape[0], 2))
+        _check(library().gs_chain(self._h, _i8(b), b.shape[0], _f8(lp), _f8(cond), _f8(amp)))
+        return lp, cond, amp[:, 0] + 1j * amp[:, 1]
+
+    def sample(self, count, seed, first=0, seeding=SEED_PLAIN):
+        """``(bits [count, ns], log2 Pr [count])`` of samples first..first+count-1 (``gsamp2 sample``/``sample2``)."""
+        bits = np.empty((count, self.ns), np.int8)
+        lp = np.empty(count)
+        _check(library().gs_sample(self._h, seed, first, count, seeding, _i8(bits), _f8(lp)))
+        return bits, lp
+
+    def hybrid(self, bits, seed, first_line=0):
+        """``(bits, log2 Pr, conditionals)``; -1 entries are sampled (``gsamp2 hybrid``)."""
+        b = self._bits(bits, allow_q=True)
+        ob = np.empty_like(b)
+        lp = np.empty(b.shape[0])
+        cond = np.empty((b.shape[0], self.ns))
+        _check(library().gs_hybrid(self._h, _i8(b), b.shape[0], seed, first_line, _i8(ob), _f8(lp), _f8(cond)))
+        return ob, lp, cond
+
+    def prefix(self, bits, K):
+        """Live table after the forced steps 0..K-1 of one string: ``(table [2^nk * ntq] complex, nk, ntq,
+        logscale, log2 Pr(prefix))`` in the dump's logical order (``gsamp2 prefix``)."""
+        b = np.ascontiguousarray(np.asarray(bits, dtype=np.int8).reshape(-1))
+        if b.size != self.ns:
+            raise ValueError("one string of ns steps")
+        nk, ntq = ctypes.c_int(), ctypes.c_int()
+        lib = library()
+        _check(lib.gs_prefix(self._h, _i8(b), K, None, 0, ctypes.byref(nk), ctypes.byref(ntq), None, None))
+        n = (1 << nk.value) * ntq.value
+        tab = np.empty(2 * n)
+        ls, lp = ctypes.c_double(), ctypes.c_double()
+        _check(lib.gs_prefix(self._h, _i8(b), K, _f8(tab), n, ctypes.byref(nk), ctypes.byref(ntq), ctypes.byref(ls),
+                             ctypes.byref(lp)))
+        return tab[0::2] + 1j * tab[1::2], nk.value, ntq.value, ls.value, lp.value
+
+    def resume(self, bits, K, tables, logscales, seed, line=0, cshift=None, tqshift=None):
+        """Sum the slice tables (logical order, with their logscales; optional twist), then run steps K..ns-1 with
+        -1 entries sampled (``gsamp2 resume``).  Returns ``(bits, log2 Pr(full), log2 Pr(prefix), cond)``."""
+        b = np.ascontiguousarray(np.asarray(bits, dtype=np.int8).reshape(-1))
+        tabs = [np.ascontiguousarray(np.stack([np.real(t), np.imag(t)], axis=1).reshape(-1)) for t in tables]
+        ptrs = (ctypes.POINTER(ctypes.c_double) * len(tabs))(*[_f8(t) for t in tabs])
+        ls = np.ascontiguousarray(np.asarray(logscales, dtype=np.float64))
+        cs = ts = None
+        if cshift is not None:
+            cs_a = np.ascontiguousarray(np.asarray(cshift, dtype=np.uint64))
+            ts_a = np.ascontiguousarray(np.asarray(tqshift, dtype=np.uint64))
+            cs = cs_a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+            ts = ts_a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        ob = np.empty(self.ns, np.int8)
+        lpf, lpp = ctypes.c_double(), ctypes.c_double()
+        cond = np.zeros(self.ns)
+        _check(library().gs_resume(self._h, _i8(b), K, len(tabs), ptrs, _f8(ls), cs, ts, seed, line, _i8(ob),
+                                   ctypes.byref(lpf), ctypes.byref(lpp), _f8(cond)))
+        return ob, lpf.value, lpp.value, cond
+
+    # profiling (engine extension, not a reference mode)
+    def profile(self, on=True):
+        _check(library().gs_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self):
+        """``(ms [ns, 3], launches [ns, 3])`` per step and sweep (gram, direct, write) since profiling was enabled."""
+        ms = np.zeros(self.ns * 3)
+        n = np.zeros(self.ns * 3, np.int64)
+        _check(library().gs_profile_read(self._h, _f8(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return ms.reshape(self.ns, 3), n.reshape(self.ns, 3)
+
+    def geometry(self, k):
+        """``(log2 cosets per thread, blocks, log2 tiles per block, ntq, route)`` of step k's sweeps."""
+        vals = [ctypes.c_int() for _ in range(5)]
+        _check(library().gs_step_geometry(self._h, k, *[ctypes.byref(v) for v in vals]))
+        return tuple(v.value for v in vals)

@@ -1,0 +1,94 @@
+/* gsamp_amd -- C ABI of the MI355X in-place cut-rank engine (libgsamp_amd.so).
+ *
+ * The reference engine has no library interface: it is the process contract of gsamp2.cpp, i.e.
+ *   gsamp2 <plan> sample|sample2 N SEED | prob STRINGS | chain STRINGS | hybrid STRINGS SEED
+ *          | prefix STRINGS K OUT | resume STRINGS K SEED DUMP... [--twist FILE]        (gsamp2.cpp:1-6, :149-225)
+ * Each entry point below replaces one of those modes; the comment of each names the reference lines whose
+ * semantics it keeps.  The bundled CLI (bin/gsamp2_amd) is a byte-compatible front end over this ABI.
+ *
+ * Conventions
+ *   - Bit strings are int8 arrays in STEP order (character k of a reference strings-file line), row-major
+ *     [nstr][ns]; values 0, 1, and -1 for '?' where a mode samples.
+ *   - All pointers are host pointers owned by the caller; every call is synchronous.
+ *   - Return value: 0 on success, else the reference's exit code for the same failure (2..7 plan parse,
+ *     8 '?' before K, 9/10 missing/bad dump, 11 allocation, 12..14 twist) or 15 (plan step structure not
+ *     supported), 16 (bad argument), 17 (HIP runtime error).  gs_last_error() gives the message of the last
+ *     failure on the calling thread.
+ *   - A zero-probability branch gives log2 Pr = -inf and zero conditionals from that step on, as the reference.
+ *   - One engine owns one device arena of 2^R entries and one HIP stream; engines are not re-entrant, separate
+ *     host threads may drive separate engines.
+ */
+#ifndef GSAMP_AMD_H
+#define GSAMP_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gs_plan gs_plan;
+typedef struct gs_engine gs_engine;
+
+enum { GS_C64 = 0, GS_C128 = 1 };          /* arena storage: complex<float> / complex<double> (-DGS_DOUBLE) */
+enum { GS_SEED_PLAIN = 0, GS_SEED_SEQ = 1 };/* sample: mt19937_64(seed+i) / sample2: seed_seq{seed,i,0x9E3779B9} */
+
+/* Parse an in-place plan file (gsamp2.cpp:126-139; exit codes 2..7) or the same text from memory. */
+int gs_plan_load(const char* path, gs_plan** out);
+int gs_plan_parse(const char* text, gs_plan** out);
+/* Plan header: qubits n, steps ns, arena bits R. */
+int gs_plan_info(const gs_plan* plan, int* n, int* ns, int* R);
+void gs_plan_destroy(gs_plan* plan);
+
+/* Allocate the 2^R arena on a HIP device and upload the plan (gsamp2.cpp:140-147; 11 when it does not fit). */
+int gs_engine_create(const gs_plan* plan, int storage, int device, gs_engine** out);
+void gs_engine_destroy(gs_engine* eng);
+
+/* prob mode (gsamp2.cpp:217-224 with chain() at :39-55): log2 Pr of each forced string, two passes per step,
+ * table renormalised every step.  bits: [nstr][ns] in {0,1}. */
+int gs_prob(gs_engine* eng, const int8_t* bits, int64_t nstr, double* log2p);
+
+/* chain mode (gsamp2.cpp:56-70, :218-222): one pass per step; log2p[i], cond[i][ns] = Pr(b_k | b_<k).
+ * amp_reim (may be NULL) receives [nstr][2]: Z(w_b) = sum_x prod_j M_j[b_j,x_j] (-1)^{x^T A x}, the last live
+ * arena entry, so that log2 Pr = log2|Z|^2 - ns (up to the global phase the graph form drops). */
+int gs_chain(gs_engine* eng, const int8_t* bits, int64_t nstr, double* log2p, double* cond, double* amp_reim);
+
+/* sample / sample2 modes (gsamp2.cpp:149-161): samples first..first+count-1, each from its own generator
+ * (seeding GS_SEED_PLAIN: mt19937_64(seed + i); GS_SEED_SEQ: std::seed_seq as gsamp2.cpp:159).
+ * bits: [count][ns] output; log2p: [count]. */
+int gs_sample(gs_engine* eng, uint64_t seed, int64_t first, int64_t count, int seeding, int8_t* bits, double* log2p);
+
+/* hybrid mode (gsamp2.cpp:99-120, :209-216): bits [nstr][ns] with -1 = '?' (sampled from the exact
+ * conditional); line i uses mt19937_64(seed + first_line + i).  out_bits [nstr][ns], log2p [nstr],
+ * cond [nstr][ns]. */
+int gs_hybrid(gs_engine* eng, const int8_t* bits, int64_t nstr, uint64_t seed, int64_t first_line,
+              int8_t* out_bits, double* log2p, double* cond);
+
+/* prefix mode (gsamp2.cpp:162-175): run forced steps 0..K-1 of one string (bits [ns], entries < K used) and
+ * return the live table of step K in logical order (c-major, sorted coset offsets of step K; {0} past the end)
+ * as complex doubles table_reim [2 * 2^nk * ntq] (capacity table_cap complex entries; the reference's dump
+ * stores the arena's type, the caller narrows), plus nk, ntq, logscale (true = stored * 2^logscale) and
+ * log2 Pr(prefix).  Pass table_reim = NULL to query nk and ntq only (no steps are run). */
+int gs_prefix(gs_engine* eng, const int8_t* bits, int K, double* table_reim, int64_t table_cap, int* nk, int* ntq,
+              double* logscale, double* log2p_prefix);
+
+/* resume mode (gsamp2.cpp:176-208) for one line: sum nslices dumped step-K tables (each [2^nk * ntq] complex
+ * doubles in logical order, with logscales ls[s]) with weights 2^(ls_s - max ls), optionally translated by
+ * the twist (cshift[s], tqshift[s]; NULL = none), scatter into the arena, then run steps K..ns-1 with '?'
+ * sampled from mt19937_64(seed + line).  out_bits [ns] (prefix copied from bits), log2p_full, log2p_prefix,
+ * cond [ns] (entries K..ns-1 written). */
+int gs_resume(gs_engine* eng, const int8_t* bits, int K, int nslices, const double* const* tables,
+              const double* logscales, const uint64_t* cshift, const uint64_t* tqshift, uint64_t seed, int64_t line,
+              int8_t* out_bits, double* log2p_full, double* log2p_prefix, double* cond);
+
+/* Timing of the device work of the last call, in milliseconds (HIP events around the step kernels). */
+double gs_last_device_ms(const gs_engine* eng);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* gs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSAMP_AMD_H */

@@ -1,0 +1,161 @@
+"""Writes the golden fixtures under tests/golden/ (run in the build container, where the reference tree and the
+reference executor compiled from its own sources by oracle/Makefile exist; the fixtures are data and travel with the
+repository, the reference does not).
+
+* ibm/    -- IBM's 70-qubit instance and its committed plans and known answers, copied as data:
+             cloud_nq76/nq70.qasm, orders/order_w30.json, cloud_ibmxeb/nq70_inplace_plan.txt, cloud_ibmxeb/nq70_w30_plan.txt,
+             cloud/nq70_plan.txt + cloud/nq70_step_qubits.json, results/nq70_graph_adjacency.txt,
+             cloud_ibmxeb/gate_strings.txt with the gate values of cloud_ibmxeb/run.sh:19, and the first 3 samples of
+             results/laptop_samples_w30.txt (gsamp2 sample 3 2000000);
+* doped/  -- data_doped/nq70_depth70_checks27_doped.qasm, cloud_doped/doped_m20_step_vertices.json and
+             cloud_doped/doped_m20_inplace_plan.txt (538 vertices, R = 34, all five step kinds);
+* c1/     -- the n = 24, band 10 instance planned by this package (plan.txt), 64 uniform strings, 8 hybrid lines,
+             and the reference executor's outputs for every mode in complex64 and complex128;
+* small/  -- 12 random small circuits (n <= 9) planned by this package with every string of each, and the
+             reference executor's prob / chain outputs (complex128).
+"""
+import json
+import os
+import shutil
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.environ.get("GS_REFERENCE", "/root/reference")
+REFBIN = os.path.join(ROOT, "oracle", "_ref")
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+from _pkg import egs  # noqa: E402
+
+GATE_VALUES = [-69.508530214929, -68.876165103181]   # cloud_ibmxeb/run.sh:19
+
+
+def run(binary, args, c128=False):
+    exe = os.path.join(REFBIN, "gsamp2_c128" if c128 else "gsamp2")
+    res = subprocess.run([exe] + args, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="4"))
+    if res.returncode:
+        raise RuntimeError(f"{exe} {args}: exit {res.returncode} {res.stderr}")
+    return res.stdout
+
+
+def copy(src, dst):
+    shutil.copyfile(os.path.join(REF, src), dst)
+
+
+def ibm():
+    d = os.path.join(HERE, "ibm")
+    os.makedirs(d, exist_ok=True)
+    copy("cloud_nq76/nq70.qasm", os.path.join(d, "nq70.qasm"))
+    copy("orders/order_w30.json", os.path.join(d, "order_w30.json"))
+    copy("cloud_ibmxeb/nq70_inplace_plan.txt", os.path.join(d, "nq70_inplace_plan.txt"))
+    copy("cloud_ibmxeb/nq70_w30_plan.txt", os.path.join(d, "nq70_w30_plan.txt"))
+    copy("cloud/nq70_plan.txt", os.path.join(d, "nq70_w31_plan.txt"))
+    copy("cloud/nq70_step_qubits.json", os.path.join(d, "nq70_w31_step_qubits.json"))
+    copy("results/nq70_graph_adjacency.txt", os.path.join(d, "nq70_graph_adjacency.txt"))
+    copy("cloud_ibmxeb/gate_strings.txt", os.path.join(d, "gate_strings.txt"))
+    with open(os.path.join(d, "gate_values.txt"), "w") as f:
+        f.write("\n".join("%.12f" % v for v in GATE_VALUES) + "\n")
+    with open(os.path.join(REF, "results/laptop_samples_w30.txt")) as f:
+        lines = [l for l in f if len(l.split()) == 3 and set(l.split()[0]) <= set("01")][:3]
+    with open(os.path.join(d, "laptop_samples_w30.txt"), "w") as f:
+        f.writelines(lines)
+
+
+def doped():
+    d = os.path.join(HERE, "doped")
+    os.makedirs(d, exist_ok=True)
+    copy("data_doped/nq70_depth70_checks27_doped.qasm", os.path.join(d, "nq70_depth70_checks27_doped.qasm"))
+    copy("cloud_doped/doped_m20_step_vertices.json", os.path.join(d, "doped_m20_step_vertices.json"))
+    copy("cloud_doped/doped_m20_inplace_plan.txt", os.path.join(d, "doped_m20_inplace_plan.txt"))
+
+
+def c1():
+    d = os.path.join(HERE, "c1")
+    os.makedirs(d, exist_ok=True)
+    inst = egs.instances.config_instance("c1")
+    plan = os.path.join(d, "plan.txt")
+    inst.write_plan(plan)
+    inst.write_qasm(os.path.join(d, "c1.qasm"))
+    with open(os.path.join(d, "step_vertices.json"), "w") as f:
+        json.dump(inst.step_vertices, f)
+    ns = len(inst.steps)
+    rng = np.random.default_rng(101)
+    uni = rng.integers(0, 2, size=(64, ns))
+    egs.planio.write_strings(os.path.join(d, "strings.txt"), uni)
+    hy = rng.integers(0, 2, size=(8, ns)).astype(np.int8)
+    hy[:, ns - 6:] = -1
+    egs.planio.write_strings(os.path.join(d, "hybrid.txt"), hy)
+    K = ns - 6
+    for c128 in (False, True):
+        tag = "c128" if c128 else "c64"
+        outs = {
+            f"prob_{tag}.txt": ["prob", plan_rel("strings.txt", d)],
+            f"chain_{tag}.txt": ["chain", plan_rel("strings.txt", d)],
+            f"sample_{tag}.txt": ["sample", "16", "1234"],
+            f"sample2_{tag}.txt": ["sample2", "16", "1234"],
+            f"hybrid_{tag}.txt": ["hybrid", plan_rel("hybrid.txt", d), "99"],
+        }
+        for name, args in outs.items():
+            with open(os.path.join(d, name), "w") as f:
+                f.write(run("gsamp2", [plan] + args, c128))
+        # prefix then resume (one slice) on the hybrid lines
+        dump = os.path.join(d, f"dump_{tag}")
+        with open(os.path.join(d, f"prefix_{tag}.txt"), "w") as f:
+            f.write(run("gsamp2", [plan, "prefix", plan_rel("hybrid.txt", d), str(K), dump], c128))
+        with open(os.path.join(d, f"resume_{tag}.txt"), "w") as f:
+            f.write(run("gsamp2", [plan, "resume", plan_rel("hybrid.txt", d), str(K), "5", dump], c128))
+    with open(os.path.join(d, "meta.json"), "w") as f:
+        json.dump(dict(K=K, sample_seed=1234, hybrid_seed=99, resume_seed=5, n=inst.N, ns=ns, R=inst.R), f)
+
+
+def plan_rel(name, d):
+    return os.path.join(d, name)
+
+
+def small():
+    """Random Clifford circuits with the reference tests' tail (tdg h rz(TH) h, test_gsamp.py:57-68)."""
+    d = os.path.join(HERE, "small")
+    os.makedirs(d, exist_ok=True)
+    rng = np.random.default_rng(31)
+    th = 0.9553166181245092
+    cases = []
+    for i in range(12):
+        n = int(rng.integers(2, 10))
+        ops = []
+        for _ in range(int(rng.integers(10, 80))):
+            g = str(rng.choice(["h", "s", "sx", "x", "z", "cz", "cz", "cz"]))
+            ops.append((g, [int(q) for q in rng.choice(n, 2, replace=False)]) if g == "cz" else (g, [int(rng.integers(n))]))
+        tail = []
+        for q in range(n):
+            tail += [("tdg", [q]), ("h", [q]), (f"rz({th})", [q]), ("h", [q])]
+        order = [int(v) for v in rng.permutation(n)]
+        t = egs.tableau.Tableau(n)
+        for name, qs in ops:
+            t.apply(name, qs)
+        am = egs.amp.Amplitudes(n, t, tail, order)
+        R = max(1, egs.plan.width(am.steps))
+        st = egs.inplace.plan_inplace(am.A, order, R)
+        plan = os.path.join(d, f"s{i}_plan.txt")
+        egs.inplace.write_inplace_plan(plan, st, R, am.M, n)
+        egs.gadgets.write_qasm(os.path.join(d, f"s{i}.qasm"), n, ops + tail)
+        with open(os.path.join(d, f"s{i}_order.json"), "w") as f:
+            json.dump(order, f)
+        sv = [s["v"] for s in st]
+        allq = np.array([[(idx >> (n - 1 - q)) & 1 for q in range(n)] for idx in range(2 ** n)])
+        egs.planio.write_strings(os.path.join(d, f"s{i}_strings.txt"), allq[:, sv])
+        for mode in ("prob", "chain"):
+            with open(os.path.join(d, f"s{i}_{mode}_c128.txt"), "w") as f:
+                f.write(run("gsamp2", [plan, mode, os.path.join(d, f"s{i}_strings.txt")], True))
+        cases.append(dict(i=i, n=n, R=R, order=order))
+    with open(os.path.join(d, "cases.json"), "w") as f:
+        json.dump(cases, f)
+
+
+if __name__ == "__main__":
+    ibm()
+    doped()
+    c1()
+    small()
+    print("fixtures written under", HERE)

@@ -1,0 +1,90 @@
+"""The C ABI library loads and exports every entry point include/gsamp_amd.h declares; plan parsing (host only)
+returns the reference's exit codes.  No device calls."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from _pkg import egs
+
+gs = egs.gsamp
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "gsamp_amd.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(gs_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_matches_python_binding():
+    assert sorted(gs.ABI_SYMBOLS) == declared_symbols()
+
+
+def test_library_exports_every_declared_symbol():
+    lib = gs.library()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", gs.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (gs_\w+)", out))
+    assert set(declared_symbols()) <= exported
+
+
+def test_library_has_gfx950_code_object():
+    with open(gs.LIB_PATH, "rb") as f:
+        assert b"gfx950" in f.read()
+
+
+def test_plan_parse_matches_reference_exit_codes(golden):
+    good = open(os.path.join(golden, "c1", "plan.txt")).read()
+    p = gs.Plan(text=good)
+    assert (p.n, p.ns, p.R) == (24, 24, 10)
+    toks = good.split()
+    # header only -> step head missing (3); head + truncated Tkeep (4); truncated term (5); truncated matrix (6)
+    nk1 = int(toks[3 + 1])
+    cases = {2: "", 3: " ".join(toks[:3])}
+    for code, text in cases.items():
+        with pytest.raises(gs.GsError) as ex:
+            gs.Plan(text=text)
+        assert ex.value.code == code
+    # nk > 40 -> 7
+    bad = toks[:3] + [toks[3], "41"] + toks[5:]
+    with pytest.raises(gs.GsError) as ex:
+        gs.Plan(text=" ".join(bad))
+    assert ex.value.code in (4, 5, 6, 7)
+
+
+def test_plan_parse_truncations_give_codes_4_5_6(golden):
+    text = open(os.path.join(golden, "ibm", "nq70_inplace_plan.txt")).read()
+    toks = text.split()
+    # step 1 of the IBM plan: "53 1 2 2 1 2 | 1 | 0 0 0 0 0 1 1 0 | 8 doubles"; the first step starts at token 3
+    first_len = 6 + int(toks[4]) + 4 * int(toks[8]) + 8
+    s1 = 3 + first_len
+    head = toks[:s1 + 6]
+    with pytest.raises(gs.GsError) as ex:
+        gs.Plan(text=" ".join(toks[:3]).replace(toks[1], toks[1]) + " " + " ".join(toks[3:s1 + 6]))
+    assert ex.value.code == 4
+    with pytest.raises(gs.GsError) as ex:
+        gs.Plan(text=" ".join(toks[:s1 + 6 + 1 + 2]))
+    assert ex.value.code == 5
+    with pytest.raises(gs.GsError) as ex:
+        gs.Plan(text=" ".join(toks[:s1 + 6 + 1 + 8 + 3]))
+    assert ex.value.code == 6
+    assert head
+
+
+def test_ibm_and_doped_plans_parse(golden):
+    p = gs.Plan(path=os.path.join(golden, "ibm", "nq70_inplace_plan.txt"))
+    assert (p.n, p.ns, p.R) == (70, 70, 30)
+    d = gs.Plan(path=os.path.join(golden, "doped", "doped_m20_inplace_plan.txt"))
+    assert (d.n, d.ns, d.R) == (538, 538, 34)
+
+
+def test_engine_without_device_fails_loudly(golden):
+    """No HIP device in this container: creating an engine must raise, not fall back to the CPU."""
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
+        pytest.skip("a HIP device may be present")
+    with pytest.raises(gs.GsError) as ex:
+        gs.Engine(os.path.join(golden, "c1", "plan.txt"))
+    assert ex.value.code in (16, 17)

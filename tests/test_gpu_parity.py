@@ -1,0 +1,206 @@
+"""Parity of the HIP engine (through the C ABI) with the reference executor's outputs and with brute force.
+
+Golden outputs come from the reference executor compiled from its own sources (tests/golden/make_golden.py);
+tolerances: complex128 arena 1e-12 on log2 Pr and conditionals (the FP64 reference path), complex64 arena 1e-5
+(the reference's own GPU-vs-CPU bound, test_cuda.py:84; c64 rounding of two independent engines), sampled bits
+identical.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from _pkg import egs
+import dense
+
+gs = egs.gsamp
+pytestmark = pytest.mark.gpu
+
+TOL = {gs.C64: 1e-5, gs.C128: 1e-12}
+TAG = {gs.C64: "c64", gs.C128: "c128"}
+
+
+def lines(path):
+    with open(path) as f:
+        return [l.split() for l in f if l.strip()]
+
+
+def close(a, b, tol):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    both_inf = np.isinf(a) & np.isinf(b) & (np.sign(a) == np.sign(b))
+    d = np.where(both_inf, 0.0, np.abs(a - b))
+    return float(np.max(d)) if d.size else 0.0
+
+
+@pytest.fixture(scope="module")
+def c1(golden):
+    d = os.path.join(golden, "c1")
+    with open(os.path.join(d, "meta.json")) as f:
+        meta = json.load(f)
+    ns = meta["ns"]
+    strings = egs.planio.read_strings(os.path.join(d, "strings.txt"), ns)
+    hy = egs.planio.read_strings(os.path.join(d, "hybrid.txt"), ns, allow_q=True)
+    return d, meta, strings, hy
+
+
+@pytest.mark.parametrize("storage", [gs.C64, gs.C128])
+def test_c1_prob_chain(c1, storage):
+    d, meta, strings, _ = c1
+    print(f"c1 prob/chain storage={TAG[storage]}", flush=True)
+    with gs.Engine(os.path.join(d, "plan.txt"), storage) as e:
+        lp = e.prob(strings)
+        ref = np.array([float(l[0]) for l in lines(os.path.join(d, f"prob_{TAG[storage]}.txt"))])
+        assert close(lp, ref, 0) <= TOL[storage], close(lp, ref, 0)
+        lpc, cond, amp = e.chain(strings)
+        refc = lines(os.path.join(d, f"chain_{TAG[storage]}.txt"))
+        assert close(lpc, [float(l[0]) for l in refc], 0) <= TOL[storage]
+        assert close(cond, [[float(x) for x in l[1:]] for l in refc], 0) <= max(TOL[storage], 1e-9)  # printed %.9f
+        # the amplitude: log2 |Z|^2 - ns is log2 Pr
+        assert close(np.log2(np.abs(amp) ** 2) - meta["ns"], lpc, 0) <= 10 * TOL[storage]
+
+
+@pytest.mark.parametrize("storage", [gs.C64, gs.C128])
+@pytest.mark.parametrize("mode", ["sample", "sample2"])
+def test_c1_sample_bits_identical(c1, storage, mode):
+    d, meta, _, _ = c1
+    print(f"c1 {mode} storage={TAG[storage]}", flush=True)
+    with gs.Engine(os.path.join(d, "plan.txt"), storage) as e:
+        bits, lp = e.sample(16, meta["sample_seed"], seeding=gs.SEED_PLAIN if mode == "sample" else gs.SEED_SEQ)
+    ref = lines(os.path.join(d, f"{mode}_{TAG[storage]}.txt"))
+    assert [egs.planio.format_bits(b) for b in bits] == [l[0] for l in ref]
+    assert close(lp, [float(l[1]) for l in ref], 0) <= TOL[storage]
+
+
+@pytest.mark.parametrize("storage", [gs.C64, gs.C128])
+def test_c1_hybrid(c1, storage):
+    d, meta, _, hy = c1
+    print(f"c1 hybrid storage={TAG[storage]}", flush=True)
+    with gs.Engine(os.path.join(d, "plan.txt"), storage) as e:
+        bits, lp, cond = e.hybrid(hy, meta["hybrid_seed"])
+    ref = lines(os.path.join(d, f"hybrid_{TAG[storage]}.txt"))
+    assert [egs.planio.format_bits(b) for b in bits] == [l[0] for l in ref]
+    assert close(lp, [float(l[1]) for l in ref], 0) <= TOL[storage]
+    assert close(cond, [[float(x) for x in l[2:]] for l in ref], 0) <= max(TOL[storage], 1e-9)
+
+
+@pytest.mark.parametrize("storage", [gs.C64, gs.C128])
+def test_c1_prefix_resume(c1, storage):
+    d, meta, _, hy = c1
+    K = meta["K"]
+    c128 = storage == gs.C128
+    print(f"c1 prefix/resume storage={TAG[storage]}", flush=True)
+    ref_pre = lines(os.path.join(d, f"prefix_{TAG[storage]}.txt"))
+    ref_res = lines(os.path.join(d, f"resume_{TAG[storage]}.txt"))
+    with gs.Engine(os.path.join(d, "plan.txt"), storage) as e:
+        for li, row in enumerate(hy):
+            tab, nk, ntq, ls, lp = e.prefix(row, K)
+            rtab, rnk, rntq, rls = egs.planio.read_dump(os.path.join(d, f"dump_{TAG[storage]}_{li}.bin"), c128)
+            assert (nk, ntq) == (rnk, rntq)
+            assert abs(ls - rls) <= 1e-6 and abs(lp - float(ref_pre[li][1])) <= TOL[storage]
+            scale = np.max(np.abs(rtab))
+            assert np.max(np.abs(tab - rtab)) <= TOL[storage] * scale * 10
+            # resume from the reference's own dump: bits and probabilities as the reference's resume
+            bits, lpf, lpp, cond = e.resume(row, K, [rtab], [rls], seed=meta["resume_seed"], line=li)
+            assert egs.planio.format_bits(bits) == ref_res[li][0]
+            assert abs(lpf - float(ref_res[li][1])) <= TOL[storage] and abs(lpp - float(ref_res[li][2])) <= TOL[storage]
+            assert close(cond[K:], [float(x) for x in ref_res[li][3:]], 0) <= max(TOL[storage], 1e-9)
+
+
+@pytest.fixture(scope="module")
+def small_cases(golden):
+    d = os.path.join(golden, "small")
+    with open(os.path.join(d, "cases.json")) as f:
+        return d, json.load(f)
+
+
+def _small_ops(path):
+    n, ops = egs.tableau.parse(path)
+    return n, ops
+
+
+def test_small_circuits_all_strings_vs_statevector(small_cases):
+    """Every string of 12 random circuits: log2 Pr (prob and chain, c128) equals the dense statevector at 1e-12
+    relative, and equals the reference executor's outputs.  Negative control: the same plans with the rows of every
+    M swapped must disagree."""
+    d, cases = small_cases
+    worst = worst_ref = ctrl = 0.0
+    for c in cases:
+        i, n = c["i"], c["n"]
+        print(f"small circuit {i} n={n}", flush=True)
+        plan = os.path.join(d, f"s{i}_plan.txt")
+        _, ops = _small_ops(os.path.join(d, f"s{i}.qasm"))
+        p = dense.probabilities(n, ops)
+        _, _, steps = egs.planio.read_inplace_plan(plan)
+        sv = [s["v"] for s in steps]
+        strings = egs.planio.read_strings(os.path.join(d, f"s{i}_strings.txt"), n)
+        idx = [dense.step_order_index(b, sv, n) for b in strings]
+        with gs.Engine(plan, gs.C128) as e:
+            lp = e.prob(strings)
+            lpc, cond, _ = e.chain(strings)
+        got = np.exp2(lp)
+        worst = max(worst, float(np.max(np.abs(got - p[idx]))))
+        assert np.allclose(np.exp2(lpc), got, rtol=1e-12, atol=1e-15)
+        ref = np.array([float(l[0]) for l in lines(os.path.join(d, f"s{i}_prob_c128.txt"))])
+        finite = np.isfinite(ref)
+        worst_ref = max(worst_ref, close(lp[finite], ref[finite], 0))
+        # negative control: M rows swapped
+        text = open(plan).read()
+        bad = _swap_rows(text)
+        with gs.Engine(gs.Plan(text=bad), gs.C128) as e:
+            ctrl = max(ctrl, float(np.max(np.abs(np.exp2(e.prob(strings)) - p[idx]))))
+    print(f"max |p - p_sv| = {worst:.2e}; max |log2 - reference| = {worst_ref:.2e}; control = {ctrl:.3f}")
+    assert worst < 1e-12 and worst_ref < 1e-10 and ctrl > 1e-2
+
+
+def _swap_rows(text):
+    toks = text.split()
+    n, ns, R = int(toks[0]), int(toks[1]), int(toks[2])
+    pos = 3
+    out = toks[:3]
+    for _ in range(ns):
+        head = toks[pos:pos + 6]
+        nk, nt = int(head[1]), int(head[5])
+        pos2 = pos + 6 + nk + 4 * nt
+        m = toks[pos2:pos2 + 8]
+        out += toks[pos:pos2] + m[4:8] + m[0:4]
+        pos = pos2 + 8
+    return " ".join(out) + "\n"
+
+
+@pytest.mark.parametrize("storage", [gs.C64, gs.C128])
+def test_ibm_gate_strings(golden, storage):
+    """IBM #151, width-30 in-place plan: the two gate strings of cloud_ibmxeb/run.sh:16-25 within 1e-5."""
+    d = os.path.join(golden, "ibm")
+    print(f"ibm gate storage={TAG[storage]}", flush=True)
+    want = [float(l[0]) for l in lines(os.path.join(d, "gate_values.txt"))]
+    with gs.Engine(os.path.join(d, "nq70_inplace_plan.txt"), storage) as e:
+        strings = egs.planio.read_strings(os.path.join(d, "gate_strings.txt"), e.ns)
+        lp = e.prob(strings)
+        lpc, cond, _ = e.chain(strings)
+    assert close(lp, want, 0) < 1e-5, lp
+    assert close(lpc, want, 0) < 1e-5, lpc
+    assert close(np.sum(np.log2(cond), axis=1), lpc, 0) < 1e-6
+
+
+def test_ibm_laptop_samples_bit_exact(golden):
+    """KAT: gsamp2 ... sample 3 2000000 (results/laptop_samples_w30.txt): identical bits, log2 Pr within 1e-5."""
+    d = os.path.join(golden, "ibm")
+    ref = lines(os.path.join(d, "laptop_samples_w30.txt"))
+    print("ibm laptop KAT sample 3 2000000", flush=True)
+    with gs.Engine(os.path.join(d, "nq70_inplace_plan.txt"), gs.C64) as e:
+        bits, lp = e.sample(3, 2000000)
+    assert [egs.planio.format_bits(b) for b in bits] == [l[0] for l in ref]
+    assert close(lp, [float(l[1]) for l in ref], 0) < 1e-5
+
+
+def test_plan_errors_from_engine(golden):
+    """Malformed plans give the reference's exit codes."""
+    good = open(os.path.join(golden, "c1", "plan.txt")).read()
+    toks = good.split()
+    with pytest.raises(gs.GsError) as ex:
+        gs.Plan(text="")
+    assert ex.value.code == 2
+    with pytest.raises(gs.GsError) as ex:
+        gs.Plan(text=" ".join(toks[:3] + toks[3:5]))
+    assert ex.value.code == 3
