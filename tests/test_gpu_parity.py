@@ -142,8 +142,10 @@ def test_small_circuits_all_strings_vs_statevector(small_cases):
         worst = max(worst, float(np.max(np.abs(got - p[idx]))))
         assert np.allclose(np.exp2(lpc), got, rtol=1e-12, atol=1e-15)
         ref = np.array([float(l[0]) for l in lines(os.path.join(d, f"s{i}_prob_c128.txt"))])
-        finite = np.isfinite(ref)
-        worst_ref = max(worst_ref, close(lp[finite], ref[finite], 0))
+        # log2 Pr is compared where the probability is not a rounding-noise zero (the reference tests do the same,
+        # test_chain.py:122-124); zero-probability strings give unrelated tiny values in any two FP64 orders
+        live = p[idx] > 1e-10
+        worst_ref = max(worst_ref, close(lp[live], ref[live], 0))
         # negative control: M rows swapped
         text = open(plan).read()
         bad = _swap_rows(text)
