@@ -41,11 +41,13 @@ def sweep_bytes(step, sweep, esz):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--c128-steps", type=int, default=8)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="first run one plain device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the in-place arena of one instance lives on one device (asked for "
@@ -54,6 +56,10 @@ def main():
 
     from _pkg import egs
     gs = egs.gsamp
+    if args.calibrate_copy:
+        ms = gs.copy_probe(args.calibrate_copy << 20)
+        print(f"copy probe: {args.calibrate_copy} MiB in {ms:.3f} ms = {2 * (args.calibrate_copy << 20) / ms / 1e6:.1f} GB/s "
+              f"(read + write)", file=sys.stderr, flush=True)
     inst = egs.instances.config_instance("c2")
     tmp = tempfile.mkdtemp(prefix="gs_bench_")
     plan_path = os.path.join(tmp, "c2_plan.txt")
@@ -95,9 +101,11 @@ def main():
     key, dom = max(agg.items(), key=lambda kv: kv[1]["ms"])
     achieved = dom["bytes"] / (dom["ms"] / 1e3) / 1e9
     v, ntq, route, sw = key
-    kname = f"sweep_kernel<float2, V={1 << v}, NTQ={ntq}, {ROUTES[route]}, {SWEEP_NAMES[sw]}>"
+    kname = f"gs::sweep_kernel<float2, {1 << v}, {ntq}, {route}, {sw}>"
+    kdesc = f"V={1 << v} cosets/thread, {ntq} slots/coset, route {ROUTES[route]}, {SWEEP_NAMES[sw]} sweep"
+    traffic_meas = profiled_traffic(kname)
     total_sweep_ms = float(ms.sum())
-    print(f"dominant kernel {kname}: {dom['n']} launches, {dom['ms'] / dom['n']:.4f} ms avg, {achieved:.1f} GB/s; "
+    print(f"dominant kernel {kname} ({kdesc}): {dom['n']} launches, {dom['ms'] / dom['n']:.4f} ms avg, {achieved:.1f} GB/s; "
           f"all sweeps {total_sweep_ms / K:.3f} ms/sample of {1e3 * s_per_sample:.3f} ms wall", file=sys.stderr)
 
     # complex128 comparison (same seeds) and c64 bits re-scored in c128
@@ -135,7 +143,8 @@ def main():
         "hbm_gbps_algorithmic": traffic["sample"] / s_per_sample / 1e9,
         "device_ms_per_sample": dev_ms / K,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kname,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_meas, "kernel": kname, "kernel_desc": kdesc,
+                     "traffic_source": TRAFFIC_PROFILE if traffic_meas else None,
                      "launches": dom["n"], "avg_ms": dom["ms"] / dom["n"],
                      "bytes_per_launch": dom["bytes"] / dom["n"]},
         "c64_vs_c128": {"c128_s_per_sample": wall2, "speedup": wall2 / s_per_sample,
@@ -143,6 +152,22 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+TRAFFIC_PROFILE = "profiles/sweep_c2_hbm_traffic.md"
+
+
+def profiled_traffic(kname):
+    """HBM bytes per dispatch of kernel kname (read + write) from the committed rocprofv3 counter summary, or None."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 4 and cells[0].strip("`") == kname:
+                return (float(cells[2]) + float(cells[3])) * 1e6
+    return None
 
 
 def cpu_baseline(plan_path, seed):

@@ -273,6 +273,13 @@ __global__ void sum_parts_kernel(const double* __restrict__ part, int nparts, do
     if (threadIdx.x == 0) out[0] = r[0];
 }
 
+// Plain streaming copy (calibration of hardware byte counters against a kernel of known traffic).
+__global__ __launch_bounds__(kThreads) void copy_probe_kernel(const float4* __restrict__ a, float4* __restrict__ b,
+                                                              int64_t n) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        b[i] = a[i];
+}
+
 int route_and_relabel(const StepInfo& si, KStep& ks, std::string& err) {
     // canonical routing (see Route): reorder the offsets so that the term routing is one of four fixed patterns
     const int n = si.ntq;
@@ -981,6 +988,37 @@ int gs_resume(gs_engine* eng, const int8_t* bits, int K, int nslices, const doub
 }
 
 double gs_last_device_ms(const gs_engine* eng) { return eng ? eng->last_ms : 0.0; }
+
+// Copy `bytes` (a multiple of 16) between two fresh device buffers with one launch of a plain streaming kernel; *ms
+// receives its duration.  Not a reference mode: it calibrates byte counters (read `bytes`, write `bytes`).
+int gs_copy_probe(int device, int64_t bytes, double* ms) {
+    if (bytes <= 0 || bytes % 16) return fail(16, "bytes must be a positive multiple of 16");
+    HIP_TRY(hipSetDevice(device));
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, size_t(bytes)) != hipSuccess || hipMalloc(&b, size_t(bytes)) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(11, "copy probe allocation failed");
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipMemset(a, 1, size_t(bytes)));
+    HIP_TRY(hipDeviceSynchronize());
+    const int64_t n = bytes / 16;
+    HIP_TRY(hipEventRecord(e0, nullptr));
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(2048), dim3(kThreads), 0, nullptr, static_cast<const float4*>(a),
+                       static_cast<float4*>(b), n);
+    HIP_TRY(hipEventRecord(e1, nullptr));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return 0;
+}
 
 // Profiling hooks (not part of the reference's interface): per-step, per-sweep kernel time from HIP events recorded
 // around every sweep launch on the engine's stream.

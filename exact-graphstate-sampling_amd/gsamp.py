@@ -64,6 +64,7 @@ def library():
         "gs_profile_enable": (I, [P, I]),
         "gs_profile_read": (I, [P, dp, ctypes.POINTER(I64)]),
         "gs_step_geometry": (I, [P, I, ip, ip, ip, ip, ip]),
+        "gs_copy_probe": (I, [I, I64, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -84,6 +85,13 @@ def _i8(a):
 
 def _f8(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def copy_probe(nbytes, device=0):
+    """Duration (ms) of one plain device copy of nbytes (reads and writes nbytes): counter calibration."""
+    ms = ctypes.c_double()
+    _check(library().gs_copy_probe(device, nbytes, ctypes.byref(ms)))
+    return ms.value
 
 
 class Plan:

@@ -1,0 +1,65 @@
+"""Summaries of rocprofv3 output for the committed profiles.
+
+  python rocprof_summary.py stats TRACE_DIR          -> top kernels of a --kernel-trace --stats run (markdown)
+  python rocprof_summary.py traffic FETCH_DIR WRITE_DIR MIB
+                                                     -> HBM bytes per dispatch per kernel from two --pmc runs
+                                                        (FETCH_SIZE, WRITE_SIZE), each scaled by what the counter
+                                                        reads for the copy probe of MIB MiB (bench.py --calibrate-copy)
+
+Input directories hold the CSV files rocprofv3 writes with ``-f csv -o run``.
+"""
+import collections
+import csv
+import os
+import sys
+
+
+def _short(name):
+    return name.split("(gs::KStep")[0].replace("HIP_vector_type<float, 2u>", "float2") \
+        .replace("HIP_vector_type<double, 2u>", "double2").replace("void ", "")
+
+
+def stats(trace_dir, top=14):
+    rows = list(csv.DictReader(open(os.path.join(trace_dir, "run_kernel_stats.csv"))))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = ["| kernel | calls | total ms | avg us | % |", "|---|---|---|---|---|"]
+    for r in rows[:top]:
+        out.append(f"| `{_short(r['Name'])[:100]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                   f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    return "\n".join(out)
+
+
+def _counter(path, cname):
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] == cname:
+            d[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return d
+
+
+def traffic(fetch_dir, write_dir, mib):
+    f = _counter(os.path.join(fetch_dir, "run_counter_collection.csv"), "FETCH_SIZE")
+    w = _counter(os.path.join(write_dir, "run_counter_collection.csv"), "WRITE_SIZE")
+    probe = next(k for k in f if "copy_probe" in k)
+    kib = mib * 1024.0
+    ff = kib / f[probe][0]
+    wf = kib / w[probe][0]
+    out = [f"calibration (copy probe, {mib} MiB read + {mib} MiB written): FETCH_SIZE reads {f[probe][0]:.0f} KiB -> "
+           f"factor {ff:.4f}; WRITE_SIZE reads {w[probe][0]:.0f} KiB -> factor {wf:.4f}",
+           "", "| kernel | dispatches | HBM read MB/dispatch | HBM write MB/dispatch |", "|---|---|---|---|"]
+    for k in f:
+        if k == probe:
+            continue
+        rd = sum(f[k]) / len(f[k]) * ff * 1024 / 1e6
+        wr = sum(w.get(k, [0])) / max(1, len(w.get(k, []))) * wf * 1024 / 1e6
+        out.append(f"| `{_short(k)[:100]}` | {len(f[k])} | {rd:.2f} | {wr:.2f} |")
+    return "\n".join(out)
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "stats":
+        print(stats(sys.argv[2]))
+    elif sys.argv[1] == "traffic":
+        print(traffic(sys.argv[2], sys.argv[3], int(sys.argv[4])))
+    else:
+        raise SystemExit(__doc__)

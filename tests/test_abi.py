@@ -48,11 +48,11 @@ def test_plan_parse_matches_reference_exit_codes(golden):
         with pytest.raises(gs.GsError) as ex:
             gs.Plan(text=text)
         assert ex.value.code == code
-    # nk > 40 -> 7
-    bad = toks[:3] + [toks[3], "41"] + toks[5:]
+    # a complete step with nk = 41 -> 7
+    nk41 = "1 1 1 " + "0 41 2 1 0 2 " + " ".join(["1"] * 41) + " 0 0 0 0 0 1 1 0 " + " ".join(["0.5"] * 8)
     with pytest.raises(gs.GsError) as ex:
-        gs.Plan(text=" ".join(bad))
-    assert ex.value.code in (4, 5, 6, 7)
+        gs.Plan(text=nk41)
+    assert ex.value.code == 7
 
 
 def test_plan_parse_truncations_give_codes_4_5_6(golden):

@@ -1,0 +1,115 @@
+"""The CPU oracle (oracle/gs_oracle, this project's restatement of gsamp2.cpp) is pinned before it is trusted:
+identical printed outputs to the reference executor on every mode of the config-1 plan (golden files written by the
+reference compiled from its own sources), probabilities equal to dense statevectors on small circuits, the
+reference's exit codes on malformed plans, and -- where the reference build exists here -- the reference's
+base-address unit test (test_baseof.cpp)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from _pkg import egs
+import dense
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE = os.path.join(ROOT, "oracle", "gs_oracle")
+
+
+def oracle(args, c128=False, cwd=None):
+    cmd = [ORACLE] + (["--c128"] if c128 else []) + args
+    res = subprocess.run(cmd, capture_output=True, text=True, cwd=cwd, env=dict(os.environ, OMP_NUM_THREADS="4"))
+    return res.returncode, res.stdout
+
+
+@pytest.fixture(scope="module")
+def c1(golden):
+    d = os.path.join(golden, "c1")
+    return d, json.load(open(os.path.join(d, "meta.json")))
+
+
+@pytest.mark.parametrize("c128", [False, True])
+def test_oracle_equals_reference_outputs(c1, c128, tmp_path):
+    d, meta = c1
+    tag = "c128" if c128 else "c64"
+    plan = os.path.join(d, "plan.txt")
+    cases = {
+        "prob": ["prob", os.path.join(d, "strings.txt")],
+        "chain": ["chain", os.path.join(d, "strings.txt")],
+        "sample": ["sample", "16", str(meta["sample_seed"])],
+        "sample2": ["sample2", "16", str(meta["sample_seed"])],
+        "hybrid": ["hybrid", os.path.join(d, "hybrid.txt"), str(meta["hybrid_seed"])],
+    }
+    for name, args in cases.items():
+        rc, out = oracle([plan] + args, c128)
+        assert rc == 0
+        ref = open(os.path.join(d, f"{name}_{tag}.txt")).read()
+        got, want = [l.split() for l in out.splitlines()], [l.split() for l in ref.splitlines()]
+        assert len(got) == len(want)
+        for g, w in zip(got, want):
+            assert g[0] == w[0] or abs(float(g[0]) - float(w[0])) < 1e-10, name
+            assert all(abs(float(a) - float(b)) < 1e-10 for a, b in zip(g[1:], w[1:]) if a != b), name
+    dump = str(tmp_path / "dump")
+    rc, out = oracle([plan, "prefix", os.path.join(d, "hybrid.txt"), str(meta["K"]), dump], c128)
+    assert rc == 0
+    ref = [l.split() for l in open(os.path.join(d, f"prefix_{tag}.txt"))]
+    got = np.array([[float(x) for x in l.split()] for l in out.splitlines()])
+    assert np.allclose(got, np.array([[float(x) for x in l] for l in ref]), atol=1e-10)
+    for li in range(len(ref)):
+        a = egs.planio.read_dump(f"{dump}_{li}.bin", c128)
+        b = egs.planio.read_dump(os.path.join(d, f"dump_{tag}_{li}.bin"), c128)
+        assert a[1:3] == b[1:3] and abs(a[3] - b[3]) < 1e-12      # logscale: FP64 sums in another order
+        assert np.allclose(a[0], b[0], rtol=0, atol=1e-6 * max(1, np.abs(b[0]).max()))
+    rc, out = oracle([plan, "resume", os.path.join(d, "hybrid.txt"), str(meta["K"]), str(meta["resume_seed"]), dump],
+                     c128)
+    assert rc == 0
+    ref = [l.split() for l in open(os.path.join(d, f"resume_{tag}.txt"))]
+    for g, w in zip([l.split() for l in out.splitlines()], ref):
+        assert g[0] == w[0] and np.allclose([float(x) for x in g[1:]], [float(x) for x in w[1:]], atol=1e-9)
+
+
+def test_oracle_small_circuits_vs_statevector(golden):
+    d = os.path.join(golden, "small")
+    cases = json.load(open(os.path.join(d, "cases.json")))
+    worst = 0.0
+    for c in cases:
+        i, n = c["i"], c["n"]
+        _, ops = egs.tableau.parse(os.path.join(d, f"s{i}.qasm"))
+        p = dense.probabilities(n, ops)
+        _, _, steps = egs.planio.read_inplace_plan(os.path.join(d, f"s{i}_plan.txt"))
+        sv = [s["v"] for s in steps]
+        strings = egs.planio.read_strings(os.path.join(d, f"s{i}_strings.txt"), n)
+        rc, out = oracle([os.path.join(d, f"s{i}_plan.txt"), "prob", os.path.join(d, f"s{i}_strings.txt")], True)
+        assert rc == 0
+        got = np.exp2([float(l) for l in out.split()])
+        want = np.array([p[dense.step_order_index(b, sv, n)] for b in strings])
+        worst = max(worst, float(np.max(np.abs(got - want))))
+    assert worst < 1e-12
+
+
+def test_oracle_exit_codes(golden, tmp_path):
+    toks = open(os.path.join(golden, "c1", "plan.txt")).read().split()
+    s = tmp_path / "strings.txt"
+    s.write_text("0" * 24 + "\n")
+    nk41 = "1 1 1 " + "0 41 2 1 0 2 " + " ".join(["1"] * 41) + " 0 0 0 0 0 1 1 0 " + " ".join(["0.5"] * 8)
+    cases = {2: "", 3: " ".join(toks[:3]), 7: nk41}
+    for code, text in cases.items():
+        p = tmp_path / f"bad{code}.txt"
+        p.write_text(text)
+        rc, _ = oracle([str(p), "prob", str(s)])
+        assert rc == code, (code, rc)
+    rc, _ = oracle([os.path.join(golden, "c1", "plan.txt")])
+    assert rc == 1
+    q = tmp_path / "q.txt"
+    q.write_text("?" + "0" * 23 + "\n")
+    rc, _ = oracle([os.path.join(golden, "c1", "plan.txt"), "prefix", str(q), "4", str(tmp_path / "x")])
+    assert rc == 8
+
+
+def test_reference_base_of_unit_test():
+    exe = os.path.join(ROOT, "oracle", "_ref", "test_baseof")
+    if not os.path.exists(exe):
+        pytest.skip("reference executor not built here")
+    res = subprocess.run([exe], capture_output=True, text=True)
+    assert res.returncode == 0, res.stdout

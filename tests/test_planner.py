@@ -1,0 +1,315 @@
+"""Host planning layer: bit-exact against the reference's committed plans, and correct against brute force.
+
+Gates (byte-identical files from the committed inputs, SURVEY build plan step 2):
+  * IBM #151 width-30 in-place plan   (cloud_nq76/nq70.qasm + orders/order_w30.json -> cloud_ibmxeb/nq70_inplace_plan.txt)
+  * two-table plans                    (cloud_ibmxeb/nq70_w30_plan.txt, cloud/nq70_plan.txt)
+  * graph-form adjacency               (results/nq70_graph_adjacency.txt)
+  * doped 538-vertex in-place plan     (T-gadget conversion of data_doped/...doped.qasm -> cloud_doped/doped_m20_inplace_plan.txt)
+Brute-force checks follow the reference tests (test_tableau.py, test_graphform.py, test_amp.py, test_plan.py,
+test_inplace.py), each with a negative control.
+"""
+import itertools
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from _pkg import egs
+import dense
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import inplace_ref  # noqa: E402
+
+TH = 0.9553166181245092
+
+
+def _read(path):
+    with open(path) as f:
+        return f.read()
+
+
+@pytest.fixture(scope="module")
+def ibm(golden):
+    d = os.path.join(golden, "ibm")
+    n, T, tail = egs.tableau.clifford_body(os.path.join(d, "nq70.qasm"))
+    order = json.load(open(os.path.join(d, "order_w30.json")))
+    am = egs.amp.Amplitudes(n, T, tail, order)
+    return d, n, T, tail, order, am
+
+
+def test_ibm_inplace_plan_byte_identical(ibm, tmp_path):
+    d, n, T, tail, order, am = ibm
+    R = egs.plan.width(am.steps)
+    assert R == 30
+    st = egs.inplace.plan_inplace(am.A, order, R, keep_order="logical")
+    out = tmp_path / "p.txt"
+    egs.inplace.write_inplace_plan(str(out), st, R, am.M, n)
+    assert _read(out) == _read(os.path.join(d, "nq70_inplace_plan.txt"))
+
+
+def test_keep_order_changes_layout_not_results():
+    """The current planner sorts kept coordinates by physical column (inplace.py:50); the committed plans were
+    written in logical order.  Both layouts are race free and give the same conditionals on every path (the layout
+    is a free injective linear choice, inplace.py:8-11)."""
+    rng = np.random.default_rng(5)
+    inst = egs.instances.Instance(12, 5, 3)
+    a = egs.inplace.plan_inplace(inst.am.A, inst.order, inst.R, keep_order="logical")
+    b = egs.inplace.plan_inplace(inst.am.A, inst.order, inst.R, keep_order="physical")
+    assert any(sa["Tkeep"] != sb["Tkeep"] for sa, sb in zip(a, b))
+    egs.inplace.check_race_free(a)
+    egs.inplace.check_race_free(b)
+    for _ in range(8):
+        path = [int(x) for x in rng.integers(0, 2, len(a))]
+        pa = inplace_ref.sample_chain_probs(a, inst.R, inst.am.M, path)
+        pb = inplace_ref.sample_chain_probs(b, inst.R, inst.am.M, path)
+        assert np.allclose(pa, pb, rtol=1e-12, atol=1e-15)
+
+
+def test_two_table_plans_byte_identical(ibm, tmp_path, golden):
+    d, n, T, tail, order, am = ibm
+    out = tmp_path / "w30.txt"
+    egs.engine.write_sampler_plan(str(out), am)
+    assert _read(out) == _read(os.path.join(d, "nq70_w30_plan.txt"))
+    o31 = json.load(open(os.path.join(d, "nq70_w31_step_qubits.json")))
+    am31 = egs.amp.Amplitudes(n, T, tail, o31)
+    out31 = tmp_path / "w31.txt"
+    egs.engine.write_sampler_plan(str(out31), am31)
+    assert _read(out31) == _read(os.path.join(d, "nq70_w31_plan.txt"))
+
+
+def test_graph_adjacency_identical(ibm):
+    d, n, T, tail, order, am = ibm
+    adj = np.array([[int(ch) for ch in l.strip()] for l in open(os.path.join(d, "nq70_graph_adjacency.txt"))
+                    if l.strip()])
+    assert (adj == am.A).all()
+
+
+def test_doped_plan_byte_identical(golden, tmp_path):
+    d = os.path.join(golden, "doped")
+    nd, ops = egs.tableau.parse(os.path.join(d, "nq70_depth70_checks27_doped.qasm"))
+    N, body, tail, anc = egs.gadgets.convert(nd, ops)
+    assert N == 538 and len(anc) == 468
+    q = tmp_path / "doped.qasm"
+    egs.gadgets.write_qasm(str(q), N, body + tail)
+    NN, TT, tl = egs.tableau.clifford_body(str(q))
+    sv = json.load(open(os.path.join(d, "doped_m20_step_vertices.json")))
+    am = egs.amp.Amplitudes(NN, TT, tl, sv)
+    st = egs.inplace.plan_inplace(am.A, sv, 34, keep_order="logical")
+    kinds = {(len(set(s["Tq"].values())), s["m"]) for s in st}
+    assert kinds == {(1, 2), (1, 1), (2, 2), (2, 1), (4, 2)}       # all five step kinds
+    out = tmp_path / "doped_plan.txt"
+    egs.inplace.write_inplace_plan(str(out), st, 34, am.M, NN)
+    assert _read(out) == _read(os.path.join(d, "doped_m20_inplace_plan.txt"))
+
+
+def _random_circuit(rng, n, nops, gates=("h", "s", "sdg", "sx", "sxdg", "x", "z", "cz", "cz")):
+    ops = []
+    for _ in range(nops):
+        g = gates[rng.integers(len(gates))]
+        ops.append((g, [int(q) for q in rng.choice(n, 2, replace=False)]) if g == "cz" else (g, [int(rng.integers(n))]))
+    return ops
+
+
+def _pauli(xr, zr, sign, n):
+    X = np.array([[0, 1], [1, 0]])
+    Z = np.diag([1, -1])
+    Y = 1j * X @ Z
+    op = np.array([[1]])
+    for q in range(n):
+        op = np.kron(op, {(0, 0): np.eye(2), (1, 0): X, (0, 1): Z, (1, 1): Y}[(int(xr[q]), int(zr[q]))])
+    return (-1) ** int(sign) * op
+
+
+def test_tableau_stabilizes_statevector():
+    """Every generator stabilizes C|0> with its sign (test_tableau.py); control: one flipped sign fails."""
+    rng = np.random.default_rng(1)
+    worst = 0.0
+    for _ in range(120):
+        n = int(rng.integers(2, 6))
+        ops = _random_circuit(rng, n, int(rng.integers(1, 40)))
+        t = egs.tableau.Tableau(n)
+        for g, qs in ops:
+            t.apply(g, qs)
+        psi = dense.simulate(n, ops)
+        worst = max(worst, max(np.linalg.norm(_pauli(t.xs[i], t.zs[i], t.r[i], n) @ psi - psi) for i in range(n)))
+    t = egs.tableau.Tableau(4)
+    ops = [("h", [0]), ("cz", [0, 1]), ("sx", [2]), ("s", [3]), ("h", [3])]
+    for g, qs in ops:
+        t.apply(g, qs)
+    t.r[0] ^= 1
+    psi = dense.simulate(4, ops)
+    ctrl = max(np.linalg.norm(_pauli(t.xs[i], t.zs[i], t.r[i], 4) @ psi - psi) for i in range(4))
+    assert worst < 1e-12 and ctrl > 1
+
+
+def _graph_state(A):
+    n = len(A)
+    v = np.empty(2 ** n, complex)
+    for idx in range(2 ** n):
+        x = [(idx >> (n - 1 - q)) & 1 for q in range(n)]
+        ph = sum(int(A[i][j]) * x[i] * x[j] for i in range(n) for j in range(i + 1, n)) & 1
+        v[idx] = (-1) ** ph
+    return v / np.sqrt(2 ** n)
+
+
+def test_graph_form_is_lc_equivalent():
+    """(x U_q)|S> = e^{i phi}|G_A> (test_graphform.py): overlap modulus 1; control: a perturbed A fails."""
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for _ in range(40):
+        n = int(rng.integers(2, 7))
+        ops = _random_circuit(rng, n, int(rng.integers(5, 50)))
+        t = egs.tableau.Tableau(n)
+        for g, qs in ops:
+            t.apply(g, qs)
+        A, U = egs.graphform.graph_form(t)
+        psi = dense.simulate(n, ops)
+        for q in range(n):
+            psi = dense.apply1(psi, U[q], q, n)
+        worst = max(worst, abs(1 - abs(np.vdot(_graph_state(A), psi))))
+        assert (A == A.T).all() and not np.diag(A).any()
+    A2 = A.copy()
+    A2[0, 1] ^= 1
+    A2[1, 0] ^= 1
+    ctrl = abs(1 - abs(np.vdot(_graph_state(A2), psi)))
+    assert worst < 1e-12 and ctrl > 1e-3
+
+
+def _brute(A, w):
+    """Z(w) = sum_x prod_j w_j(x_j) (-1)^{sum_{i<j} A_ij x_i x_j} by enumeration (ref.py:4-11 semantics)."""
+    n = len(A)
+    tot = 0j
+    for x in itertools.product((0, 1), repeat=n):
+        ph = sum(int(A[i][j]) * x[i] * x[j] for i in range(n) for j in range(i + 1, n)) & 1
+        p = (-1) ** ph
+        for j in range(n):
+            p *= w[j][x[j]]
+        tot += p
+    return tot
+
+
+def _tail_circuit(rng, n):
+    ops = _random_circuit(rng, n, int(rng.integers(10, 60)), ("h", "s", "sx", "x", "z", "cz", "cz", "cz"))
+    tail = []
+    for q in range(n):
+        tail += [("tdg", [q]), ("h", [q]), (f"rz({TH})", [q]), ("h", [q])]
+    return ops, tail
+
+
+def test_amplitudes_brute_force_vs_statevector():
+    """|2^{-n/2} Z(w_b)|^2 with w from Amplitudes.M equals the statevector probability (test_amp.py)."""
+    rng = np.random.default_rng(3)
+    worst = 0.0
+    for _ in range(10):
+        n = int(rng.integers(2, 7))
+        ops, tail = _tail_circuit(rng, n)
+        t = egs.tableau.Tableau(n)
+        for g, qs in ops:
+            t.apply(g, qs)
+        am = egs.amp.Amplitudes(n, t, tail)
+        p = dense.probabilities(n, ops + tail)
+        for idx in range(2 ** n):
+            b = [(idx >> (n - 1 - q)) & 1 for q in range(n)]
+            z = _brute(am.A, am.weights(b))
+            worst = max(worst, abs(abs(z) ** 2 / 2 ** n - p[idx]))
+    assert worst < 1e-12
+
+
+def _run_plan(steps, w):
+    """Two-table gather executor of a plan (ref.py:13-31 semantics): Z(w)."""
+    old = np.array([1.0 + 0j])
+    for st in steps:
+        r, rn, rho, piv, ker = st["r"], st["rn"], st["rho"], st["piv"], st["ker"]
+        kspan = [0]
+        for k in ker:
+            kspan += [s ^ k for s in kspan]
+        new = np.zeros(1 << rn, complex)
+        for cp in range(1 << rn):
+            u0 = sum(1 << piv[j] for j in range(rn) if (cp >> j) & 1)
+            acc = 0
+            for k in kspan:
+                u = u0 ^ k
+                c = u & ((1 << r) - 1)
+                x = (u >> r) & 1
+                acc += old[c] * w[st["v"]][x] * (-1 if (x and bin(c & rho).count("1") & 1) else 1)
+            new[cp] = acc
+        old = new
+    return old[0]
+
+
+def test_two_table_plan_executes_to_brute_force():
+    """plan(A, order) executed in gather form equals the brute-force Ising sum (test_plan.py); control: dropping
+    the rho sign fails."""
+    rng = np.random.default_rng(11)
+    worst = ctrl = 0.0
+    for _ in range(12):
+        n = int(rng.integers(2, 9))
+        A = np.triu(rng.integers(0, 2, (n, n)), 1)
+        A = (A + A.T).astype(np.uint8)
+        order = [int(v) for v in rng.permutation(n)]
+        w = [(complex(*rng.normal(size=2)), complex(*rng.normal(size=2))) for _ in range(n)]
+        st = egs.plan.plan(A, order)
+        z = _brute(A, w)
+        worst = max(worst, abs(_run_plan(st, w) - z) / max(1, abs(z)))
+        bad = [dict(s, rho=0) for s in st]
+        ctrl = max(ctrl, abs(_run_plan(bad, w) - z) / max(1, abs(z)))
+    assert worst < 1e-12 and ctrl > 1e-3
+
+
+def test_inplace_executor_matches_statevector_with_race_checks():
+    """The in-place layout: product of the conditionals along every path equals |psi|^2 with the race asserts on
+    (test_inplace.py); control: swapping the tau labels fails."""
+    rng = np.random.default_rng(31)
+    worst = ctrl = 0.0
+    ncase = 0
+    for trial in range(16):
+        n = int(rng.integers(2, 7))
+        ops, tail = _tail_circuit(rng, n)
+        t = egs.tableau.Tableau(n)
+        for g, qs in ops:
+            t.apply(g, qs)
+        order = [int(v) for v in rng.permutation(n)]
+        am = egs.amp.Amplitudes(n, t, tail, order)
+        R = max(1, egs.plan.width(am.steps))
+        steps = egs.inplace.plan_inplace(am.A, order, R)
+        egs.inplace.check_race_free(steps)
+        psi = dense.simulate(n, ops + tail)
+        for path in itertools.product((0, 1), repeat=n):
+            p = float(np.prod(inplace_ref.sample_chain_probs(steps, R, am.M, path)))
+            idx = dense.step_order_index(path, order, n)
+            worst = max(worst, abs(p - abs(psi[idx]) ** 2))
+        ncase += 1
+        if trial < 6 and any(s["m"] == 2 and s["r"] > 0 for s in steps):
+            bad = [dict(s, terms=[(qv, x, 1 - tau if s["m"] == 2 else tau) for qv, x, tau in s["terms"]]) for s in steps]
+            for path in itertools.product((0, 1), repeat=n):
+                p = float(np.prod(inplace_ref.sample_chain_probs(bad, R, am.M, path, check_races=False)))
+                ctrl = max(ctrl, abs(p - abs(psi[dense.step_order_index(path, order, n)]) ** 2))
+    assert worst < 1e-12 and ctrl > 1e-3
+
+
+def test_instances_widths_and_race_freedom():
+    for name, want_r in (("c1", 10), ("c2", 30)):
+        inst = egs.instances.config_instance(name)
+        assert inst.R == want_r and inst.width <= want_r
+        assert inst.steps[0]["r"] == 0 and not inst.steps[-1]["keep"]
+    c1 = egs.instances.config_instance("c1")
+    assert egs.inplace.check_race_free(c1.steps) > 0
+    assert len(egs.instances.clifford_words()) == 24
+
+
+def test_planio_roundtrip(tmp_path, golden):
+    bits = np.array([[0, 1, -1, 1], [1, 1, 0, 0]], np.int8)
+    p = tmp_path / "s.txt"
+    egs.planio.write_strings(str(p), bits)
+    assert (egs.planio.read_strings(str(p), 4, allow_q=True) == bits).all()
+    n, R, steps = egs.planio.read_inplace_plan(os.path.join(golden, "ibm", "nq70_inplace_plan.txt"))
+    assert (n, R, len(steps)) == (70, 30, 70)
+    t = egs.planio.plan_traffic(steps, 8)
+    assert abs(np.log2(t["sample"] / 8 / 3) - 33.0) < 0.01        # sum 2^r_k = 2^33.001 (3 units per sample)
+    tab = np.arange(8) + 1j
+    egs.planio.write_dump(str(tmp_path / "d.bin"), tab, 2, 2, -1.5)
+    t2, nk, ntq, ls = egs.planio.read_dump(str(tmp_path / "d.bin"))
+    assert (nk, ntq, ls) == (2, 2, -1.5) and np.allclose(t2, tab)
