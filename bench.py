@@ -26,14 +26,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
-SWEEP_NAMES = ("gram", "direct", "write")
+SWEEP_NAMES = ("gram", "direct", "write", "write+gram", "write+direct")
 ROUTES = ("m1", "split", "mixA", "mixB")
 
 
 def sweep_bytes(step, sweep, esz):
+    """Algorithmic bytes of one sweep launch on a step: norm sweeps read the live slots; write sweeps (also the ones
+    that fold in the next step's norms, which read nothing more) read the live slots and write the outputs."""
     ntq = len({t[0] for t in step["terms"]})
     read = (1 << step["nk"]) * ntq * esz
-    if sweep == 2:
+    if sweep >= 2:
         return read + (1 << step["nk"]) * (2 if step["m"] == 2 else 1) * esz
     return read
 
@@ -46,6 +48,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--c128-steps", type=int, default=8)
+    ap.add_argument("--no-fusion", action="store_true", help="run every norm pass as its own sweep")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="first run one plain device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -71,6 +74,8 @@ def main():
 
     K, W = args.steps, args.warmup
     with gs.Engine(plan_path, gs.C64) as e:
+        if args.no_fusion:
+            e.set_fusion(False)
         if W:
             e.sample(W, args.seed + 10 ** 6)
         e.profile(True)
@@ -85,14 +90,17 @@ def main():
         geo = [e.geometry(k) for k in range(e.ns)]
     s_per_sample = wall / K
 
-    # dominant kernel: aggregate per template instance (storage, v, ntq, route, sweep)
+    # dominant kernel: aggregate per template instance <storage, VR, VG, ntq, route, sweep, next route>
     agg = {}
     for k, s in enumerate(steps):
-        v, grid, lgi, ntq, route = geo[k]
-        for sw in range(3):
+        gk = geo[k]
+        for sw in range(5):
             if nl[k, sw] == 0:
                 continue
-            key = (v, ntq, route, sw)
+            if sw >= 3:
+                key = (1 << 1, 2, gk["ntq"], gk["route"], sw, gk["route2"])   # fused layout: runs of 2 x 2
+            else:
+                key = (1 << gk["lr"], 1, gk["ntq"], gk["route"], sw, 0)
             a = agg.setdefault(key, dict(ms=0.0, n=0, bytes=0.0, steps=[]))
             a["ms"] += ms[k, sw]
             a["n"] += int(nl[k, sw])
@@ -100,9 +108,10 @@ def main():
             a["steps"].append(k)
     key, dom = max(agg.items(), key=lambda kv: kv[1]["ms"])
     achieved = dom["bytes"] / (dom["ms"] / 1e3) / 1e9
-    v, ntq, route, sw = key
-    kname = f"gs::sweep_kernel<float2, {1 << v}, {ntq}, {route}, {sw}>"
-    kdesc = f"V={1 << v} cosets/thread, {ntq} slots/coset, route {ROUTES[route]}, {SWEEP_NAMES[sw]} sweep"
+    vr, vg, ntq, route, sw, route2 = key
+    kname = f"gs::sweep_kernel<float2, {vr}, {vg}, {ntq}, {route}, {sw}, {route2}>"
+    kdesc = (f"{vr * vg} cosets/thread ({vg} run(s) of {vr}), {ntq} slots/coset, route {ROUTES[route]}, "
+             f"{SWEEP_NAMES[sw]} sweep" + (f", next route {ROUTES[route2]}" if sw >= 3 else ""))
     traffic_meas = profiled_traffic(kname)
     total_sweep_ms = float(ms.sum())
     print(f"dominant kernel {kname} ({kdesc}): {dom['n']} launches, {dom['ms'] / dom['n']:.4f} ms avg, {achieved:.1f} GB/s; "

@@ -1,5 +1,5 @@
 // Device-side data of the engine: the per-step descriptor the sweep kernels read, the control block that carries
-// the sampling chain's state from step to step on the device, and the kernel entry points.
+// the sampling chain's state from step to step on the device, and the per-launch epilogue arguments.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -9,7 +9,7 @@ namespace gs {
 
 constexpr int kThreads = 256;       // threads per sweep block (4 waves of 64)
 constexpr int kMaxGridLog2 = 11;    // at most 2048 blocks per sweep; each loops over tiles
-constexpr int kPartStride = 4;      // doubles per block partial
+constexpr int kPartStride = 8;      // doubles per block partial (write norms + fused next-step norms)
 
 // Canonical routing of the 2 * ntq terms of a step onto the outputs tau (after the host relabels the offsets):
 //   kRoute1     m = 1: every term feeds tau 0
@@ -18,24 +18,33 @@ constexpr int kPartStride = 4;      // doubles per block partial
 //   kRouteMixB  m = 2: offset j feeds tau (j & 1) with both x
 enum Route { kRoute1 = 0, kRouteSplit = 1, kRouteMixA = 2, kRouteMixB = 3 };
 
-// Sweep kinds.
-enum Sweep { kGram = 0, kDirect = 1, kWrite = 2 };
+// Sweep kinds.  kWriteGram / kWriteDirect store step k and, from the values just stored, reduce the norms of step
+// k+1 (whose two slots per coset are both produced by one thread: the thread's cosets span the pairing vector).
+enum Sweep { kGram = 0, kDirect = 1, kWrite = 2, kWriteGram = 3, kWriteDirect = 4 };
 
-// One step as the sweep kernels see it (device memory, read with uniform loads).
+// One step as the sweep kernels see it (device memory, read with uniform loads).  Coset coordinates are in the
+// kernel's order: bits 0..lr-1 are unit physical bits (a run of consecutive slots), bit lr is the pairing coordinate
+// when vg == 2, then the rest.
 struct KStep {
-    uint64_t cols[40];     // Tkeep (physical column of each kept coordinate)
+    uint64_t cols[40];     // physical column of each coset coordinate
     uint64_t off[4];       // coset offsets, relabelled so that the routing is canonical
     uint64_t g;            // second output slot offset (m = 2)
-    uint64_t rhoc;         // sign mask over the kept coordinates
+    uint64_t pmask[3];     // parity masks over the coordinates: [0] output sign (rhoc), [1] next step's slot index,
+                           // [2] next step's sign (fused sweeps)
     double mre[2][2];      // M[b][x]
     double mim[2][2];
     uint32_t parmask;      // bit j: parity(qv_j & rho) of offset j
-    int nk;                // log2 number of cosets
-    int v;                 // log2 cosets per thread (the low v kept columns are the unit bits 0..v-1)
+    int nk;                // number of coset coordinates
+    int lr;                // log2 run length (cosets per thread along unit bits)
+    int vg;                // 1, or 2 when coordinate lr is the pairing coordinate of the next step
     int lg_grid;           // log2 blocks
     int lg_iter;           // log2 tiles per block
     int ntq, m, route;
     int growth;            // q = 0 and m = 2 with x -> tau: branch norms follow from the table norm
+    int route2;            // fused: next step's route
+    int par2;              // fused: next step's par of its second offset
+    int eps;               // fused: the partner of output tau is output tau ^ eps of the paired coset
+    int phig, nug;         // fused: slot-index and sign parity of g
 };
 
 // Chain state carried on the device between steps (one per engine).
@@ -46,14 +55,31 @@ struct Ctl {
     double logscale;       // run_steps: stored table = true table * 2^-logscale
     double sc;             // scale of the pending write
     double w[4];           // M[bv][0]*sc, M[bv][1]*sc as re, im pairs (read by the write sweep)
-    double amp[2];         // chain1: final single live entry Z
     int bv;                // decision of the pending write
     int dead;              // a zero-probability branch was hit: later kernels do nothing
     int qidx;              // index of the next pre-drawn uniform (run_steps mode)
     int pad;
 };
 
-// Modes of the chain (how the decide kernel turns norms into a branch and bookkeeping).
+// Modes of the chain (how norms become a branch and bookkeeping).
 enum ChainMode { kChain = 0, kChain1 = 1, kRun = 2 };
+
+// What the last block of a sweep does after the sweep.
+enum EpiRole { kRoleNorm = 0, kRoleWrite = 1 };
+// Write role: what follows step k.
+enum Next { kNextPublish = -2, kNextDecide = -1, kNextFusedGram = kGram, kNextFusedDirect = kDirect, kNextSweep = 3 };
+
+struct Epi {
+    int k, k1, ns, mode;
+    int role;                  // EpiRole
+    int next;                  // Next (write role)
+    const int8_t* forced;      // this string's forced bits (nullptr: every step sampled)
+    const double* U;           // this string's pre-drawn uniforms
+    int8_t* bits;              // this string's output bits
+    double* cond;              // this string's conditionals
+    double* out;               // this string's scalars: logp, logscale, prev, dead
+    unsigned* counter;         // blocks finished (the last block resets it)
+    const KStep* plain;        // plain descriptors (M, growth flags) indexed by step
+};
 
 }  // namespace gs

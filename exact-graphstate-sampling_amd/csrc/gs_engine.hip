@@ -27,8 +27,11 @@
 #include "gs_plan.h"
 
 namespace gs {
-void* sweep_kernel_for(int storage, int v, int ntq, int route, int sweep);
-int preferred_v(int storage);
+void* sweep_kernel_for(int storage, int lr, int vg, int ntq, int route, int sweep, int route2);
+int plain_lr(int storage);
+int fused_lr(int storage);
+void* decide_kernel_ptr();
+void* publish_kernel_ptr();
 }  // namespace gs
 
 using namespace gs;
@@ -50,141 +53,6 @@ int fail(int code, const std::string& msg) {
 
 // ------------------------------------------------------------------------------------------------------------
 // small kernels
-
-struct DecideArgs {
-    int k;          // step to decide; k == k1 finishes the run
-    int k0, k1;     // this run covers steps k0..k1-1
-    int ns;
-    int mode;       // ChainMode
-    int norm;       // -1 none, kGram, kDirect: which partials of step k are in part_n
-    int nparts_n;   // blocks of step k's norm sweep
-    int nparts_w;   // blocks of step k-1's write sweep (0 when k == k0)
-};
-
-// Sum nparts partials (stride kPartStride) into out[0..nq) in a fixed order: thread t sums t, t+256, ... then a
-// fixed tree.  All threads must call it.
-__device__ void reduce_parts(const double* part, int nparts, int nq, double* out, double (*sh)[4]) {
-    const int t = threadIdx.x;
-    double acc[4] = {0, 0, 0, 0};
-    for (int i = t; i < nparts; i += kThreads)
-        for (int q = 0; q < nq; q++) acc[q] += part[size_t(i) * kPartStride + q];
-    for (int q = 0; q < 4; q++) sh[t][q] = acc[q];
-    __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
-        if (t < w)
-            for (int q = 0; q < 4; q++) sh[t][q] += sh[t + w][q];
-        __syncthreads();
-    }
-    if (t == 0)
-        for (int q = 0; q < nq; q++) out[q] = sh[0][q];
-    __syncthreads();
-}
-
-__device__ double2 step_m(const KStep& s, int b, int x) { return make_double2(s.mre[b][x], s.mim[b][x]); }
-
-__global__ __launch_bounds__(kThreads) void decide_kernel(const KStep* __restrict__ steps, Ctl* __restrict__ ctl,
-                                                          const double* __restrict__ part_n,
-                                                          const double* __restrict__ part_w, DecideArgs a,
-                                                          const int8_t* __restrict__ forced,
-                                                          const double* __restrict__ U, int8_t* __restrict__ bits,
-                                                          double* __restrict__ cond, double* __restrict__ out) {
-    __shared__ double sh[kThreads][4];
-    __shared__ double red_n[4], red_w[4];
-    reduce_parts(part_n, a.norm >= 0 ? a.nparts_n : 0, 4, red_n, sh);
-    reduce_parts(part_w, a.nparts_w, 2, red_w, sh);
-    if (threadIdx.x != 0) return;
-    Ctl& c = *ctl;
-    // ---- bookkeeping of the write of step k-1
-    if (!c.dead && a.nparts_w > 0) {
-        const int kp = a.k - 1;
-        const double nb = red_w[0];
-        c.nstored = red_w[1];
-        if (a.mode == kChain1) {
-            cond[kp] = nb / (2 * c.prev);
-            c.prev = nb;
-            if (nb <= 0) {
-                for (int q = kp; q < a.ns; q++) cond[q] = 0;
-                c.logp = -INFINITY;
-                c.dead = 1;
-            }
-        } else if (a.mode == kRun) {
-            c.logscale += 0.5 * log2(c.prev);
-            cond[kp] = nb / 2;
-            c.prev = nb;
-            if (nb <= 0) {
-                for (int q = kp; q < a.k1; q++) {
-                    cond[q] = 0;
-                    bits[q] = forced[q] < 0 ? 0 : forced[q];
-                }
-                c.logp = -INFINITY;
-                c.dead = 1;
-            } else {
-                c.logp += log2(cond[kp]);
-            }
-        }
-    }
-    if (a.k == a.k1) {                                      // finish: publish the run's scalars
-        if (a.mode == kChain1 && !c.dead) c.logp = log2(c.prev) - a.ns;
-        out[0] = c.logp;
-        out[1] = c.logscale;
-        out[2] = c.prev;
-        out[3] = c.dead;
-        return;
-    }
-    if (c.dead) return;
-    // ---- decision of step k
-    const KStep& s = steps[a.k];
-    int bv = forced ? forced[a.k] : -1;
-    double sc = 1.0;
-    bool need_p = (a.mode == kChain) || (a.mode == kRun && bv < 0);
-    double n0 = 0, n1 = 0;
-    if (need_p) {
-        if (a.norm == kGram) {
-            const double g00 = red_n[0], g11 = red_n[1], gr = red_n[2], gi = red_n[3];
-            for (int b = 0; b < 2; b++) {
-                const double2 m0 = step_m(s, b, 0), m1 = step_m(s, b, 1);
-                // 2 Re(m0 conj(m1) G01)
-                const double pr = m0.x * m1.x + m0.y * m1.y, pi = m0.y * m1.x - m0.x * m1.y;
-                const double nbv = (m0.x * m0.x + m0.y * m0.y) * g00 + (m1.x * m1.x + m1.y * m1.y) * g11 +
-                                   2 * (pr * gr - pi * gi);
-                (b ? n1 : n0) = nbv;
-            }
-        } else if (a.norm == kDirect) {
-            n0 = red_n[0];
-            n1 = red_n[1];
-        } else {                                              // growth step
-            for (int b = 0; b < 2; b++) {
-                const double2 m0 = step_m(s, b, 0), m1 = step_m(s, b, 1);
-                (b ? n1 : n0) = (m0.x * m0.x + m0.y * m0.y + m1.x * m1.x + m1.y * m1.y) * c.nstored;
-            }
-        }
-    }
-    if (a.mode == kChain) {
-        const double p1 = n1 / (n0 + n1);
-        if (bv < 0) bv = U[a.k] < p1;
-        const double pb = bv ? p1 : 1 - p1;
-        if (pb <= 0) {
-            c.logp = -INFINITY;
-            c.dead = 1;
-            return;
-        }
-        c.logp += log2(pb);
-        sc = 1.0 / sqrt(bv ? n1 : n0);
-    } else if (a.mode == kChain1) {
-        sc = 1.0;
-    } else {
-        sc = 1.0 / sqrt(c.prev);
-        if (bv < 0) bv = U[c.qidx++] < n1 / (n0 + n1);
-    }
-    bits[a.k] = int8_t(bv);
-    c.bv = bv;
-    c.sc = sc;
-    const double2 m0 = step_m(s, bv, 0), m1 = step_m(s, bv, 1);
-    c.w[0] = m0.x * sc;
-    c.w[1] = m0.y * sc;
-    c.w[2] = m1.x * sc;
-    c.w[3] = m1.y * sc;
-}
 
 template <class S>
 __device__ S to_storage(double2 v);
@@ -266,11 +134,11 @@ __global__ __launch_bounds__(kThreads) void scatter_kernel(const KStep* __restri
     }
 }
 
+// Fixed-order sum of the first value of nparts partials (one thread; used once per resume).
 __global__ void sum_parts_kernel(const double* __restrict__ part, int nparts, double* __restrict__ out) {
-    __shared__ double sh[kThreads][4];
-    __shared__ double r[4];
-    reduce_parts(part, nparts, 1, r, sh);
-    if (threadIdx.x == 0) out[0] = r[0];
+    double t = 0;
+    for (int i = 0; i < nparts; i++) t += part[size_t(i) * kPartStride];
+    out[0] = t;
 }
 
 // Plain streaming copy (calibration of hardware byte counters against a kernel of known traffic).
@@ -337,17 +205,24 @@ struct gs_engine {
     size_t esz = 8;                 // bytes per arena entry
     hipStream_t stream = nullptr;
     void* arena = nullptr;
-    KStep* dsteps = nullptr;
-    std::vector<KStep> hsteps;
+    // per step: the plain descriptor (norm and write sweeps) and, where the next step's norms can be folded into
+    // this step's write, the fused descriptor (coordinates relabelled so that each thread owns the pairs)
+    std::vector<KStep> hplain, hfused;
+    std::vector<char> fusable;
+    KStep* dplain = nullptr;
+    KStep* dfused = nullptr;
     struct Launch {
         void* gram = nullptr;
         void* direct = nullptr;
         void* write = nullptr;
+        void* wgram = nullptr;      // fused write + next step's Gram
+        void* wdirect = nullptr;    // fused write + next step's direct norms
         int grid = 1;
+        int fgrid = 1;
     };
     std::vector<Launch> launch;
-    double* part_n = nullptr;
-    double* part_w = nullptr;
+    double* part = nullptr;         // per-block partials of the running sweep
+    unsigned* counter = nullptr;    // blocks finished in the running sweep
     Ctl* ctl = nullptr;
     // per-string buffers, kChunk strings
     static constexpr int kChunk = 64;
@@ -365,42 +240,47 @@ struct gs_engine {
     unsigned char* h_amp = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
+    bool allow_fusion = true;
     // optional per-sweep timing (bench / profiling)
     bool profile = false;
     std::vector<hipEvent_t> pev;
-    std::vector<int> pev_tag;       // (step << 2) | sweep
-    std::vector<double> prof_ms;    // [ns][3]
-    std::vector<long> prof_n;       // [ns][3]
+    std::vector<int> pev_tag;       // step * 8 + sweep
+    std::vector<double> prof_ms;    // [ns][kNumSweeps]
+    std::vector<long> prof_n;       // [ns][kNumSweeps]
 };
 
 namespace {
 
+constexpr int kNumSweeps = 5;
+
 int ns_of(const gs_engine* e) { return e->plan.ns; }
 
-void* arena_void(gs_engine* e) { return e->arena; }
-
-int launch_sweep(gs_engine* e, int k, int sweep) {
+int launch_sweep(gs_engine* e, int k, int sweep, const Epi& epi) {
     const auto& L = e->launch[size_t(k)];
-    void* fn = sweep == kGram ? L.gram : sweep == kDirect ? L.direct : L.write;
+    void* fn = sweep == kGram ? L.gram : sweep == kDirect ? L.direct : sweep == kWrite ? L.write
+             : sweep == kWriteGram ? L.wgram : L.wdirect;
     if (!fn) return fail(15, "no kernel for step " + std::to_string(k));
-    const KStep* sp = e->dsteps + k;
+    const bool fused = sweep == kWriteGram || sweep == kWriteDirect;
+    const KStep* sp = (fused ? e->dfused : e->dplain) + k;
+    const int grid = fused ? L.fgrid : L.grid;
     void* mem = e->arena;
-    double* part = sweep == kWrite ? e->part_w : e->part_n;
-    const Ctl* ctl = e->ctl;
-    void* args[] = {&sp, &mem, &part, &ctl};
+    double* part = e->part;
+    Ctl* ctl = e->ctl;
+    Epi ep = epi;
+    void* args[] = {&sp, &mem, &part, &ctl, &ep};
     if (e->profile) {
         hipEvent_t a, b;
         HIP_TRY(hipEventCreate(&a));
         HIP_TRY(hipEventCreate(&b));
         HIP_TRY(hipEventRecord(a, e->stream));
-        HIP_TRY(hipLaunchKernel(fn, dim3(L.grid), dim3(kThreads), args, 0, e->stream));
+        HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kThreads), args, 0, e->stream));
         HIP_TRY(hipEventRecord(b, e->stream));
         e->pev.push_back(a);
         e->pev.push_back(b);
-        e->pev_tag.push_back((k << 2) | sweep);
+        e->pev_tag.push_back(k * 8 + sweep);
         return 0;
     }
-    HIP_TRY(hipLaunchKernel(fn, dim3(L.grid), dim3(kThreads), args, 0, e->stream));
+    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kThreads), args, 0, e->stream));
     return 0;
 }
 
@@ -408,9 +288,9 @@ int collect_profile(gs_engine* e) {
     for (size_t i = 0; i < e->pev_tag.size(); i++) {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, e->pev[2 * i], e->pev[2 * i + 1]));
-        const int k = e->pev_tag[i] >> 2, sw = e->pev_tag[i] & 3;
-        e->prof_ms[size_t(k) * 3 + sw] += ms;
-        e->prof_n[size_t(k) * 3 + sw] += 1;
+        const int k = e->pev_tag[i] / 8, sw = e->pev_tag[i] % 8;
+        e->prof_ms[size_t(k) * kNumSweeps + sw] += ms;
+        e->prof_n[size_t(k) * kNumSweeps + sw] += 1;
         (void)hipEventDestroy(e->pev[2 * i]);
         (void)hipEventDestroy(e->pev[2 * i + 1]);
     }
@@ -419,15 +299,15 @@ int collect_profile(gs_engine* e) {
     return 0;
 }
 
-int launch_decide(gs_engine* e, const DecideArgs& a, const int8_t* forced, const double* U, int8_t* bits, double* cond,
-                  double* out) {
-    const KStep* steps = e->dsteps;
+int launch_small(gs_engine* e, void* fn, Epi epi, int k) {
     Ctl* ctl = e->ctl;
-    const double* pn = e->part_n;
-    const double* pw = e->part_w;
-    DecideArgs aa = a;
-    void* args[] = {&steps, &ctl, &pn, &pw, &aa, &forced, &U, &bits, &cond, &out};
-    HIP_TRY(hipLaunchKernel(reinterpret_cast<void*>(&decide_kernel), dim3(1), dim3(kThreads), args, 0, e->stream));
+    if (k >= 0) {
+        void* args[] = {&ctl, &epi, &k};
+        HIP_TRY(hipLaunchKernel(fn, dim3(1), dim3(1), args, 0, e->stream));
+    } else {
+        void* args[] = {&ctl, &epi};
+        HIP_TRY(hipLaunchKernel(fn, dim3(1), dim3(1), args, 0, e->stream));
+    }
     return 0;
 }
 
@@ -444,9 +324,9 @@ int launch_init(gs_engine* e) {
     return 0;
 }
 
-// Which norm sweep step k needs in a mode (-1: none).
+// Which norms step k needs in a mode (-1: none -- forced, chain1, or a growth step).
 int norm_for(const gs_engine* e, int mode, int k, int forced_bit, bool forced_string) {
-    const KStep& s = e->hsteps[size_t(k)];
+    const KStep& s = e->hplain[size_t(k)];
     if (mode == kChain1) return -1;
     if (mode == kRun && forced_bit >= 0) return -1;
     if (s.growth) return -1;
@@ -454,49 +334,192 @@ int norm_for(const gs_engine* e, int mode, int k, int forced_bit, bool forced_st
     return kGram;
 }
 
-// Enqueue steps k0..k1-1 of one string plus the finishing decide.  forced_h: host copy of the string's forced bits
-// (nullptr when every step is sampled); device pointers address this string's slot.
+// Enqueue steps k0..k1-1 of one string.  forced_h: host copy of the string's forced bits (nullptr when every step is
+// sampled); device pointers address this string's slot.  Per step: [norm sweep, when the previous write could not
+// fold it in] -> write sweep, whose last block does the bookkeeping and decides the next step.
 int enqueue_run(gs_engine* e, int mode, int k0, int k1, const int8_t* forced_h, const int8_t* forced_d, const double* U_d,
                 int8_t* bits_d, double* cond_d, double* out_d) {
-    int prev_grid = 0;
-    for (int k = k0; k <= k1; k++) {
-        DecideArgs a{};
-        a.k = k;
-        a.k0 = k0;
-        a.k1 = k1;
-        a.ns = e->plan.ns;
-        a.mode = mode;
-        a.norm = -1;
-        a.nparts_w = prev_grid;
-        if (k < k1) {
-            const int fb = forced_h ? forced_h[k] : -1;
-            a.norm = norm_for(e, mode, k, fb, forced_h != nullptr && mode == kChain);
-            if (a.norm >= 0) {
-                if (int rc = launch_sweep(e, k, a.norm)) return rc;
-                a.nparts_n = e->launch[size_t(k)].grid;
+    Epi base{};
+    base.k1 = k1;
+    base.ns = e->plan.ns;
+    base.mode = mode;
+    base.forced = forced_d;
+    base.U = U_d;
+    base.bits = bits_d;
+    base.cond = cond_d;
+    base.out = out_d;
+    base.counter = e->counter;
+    base.plain = e->dplain;
+    if (k0 == k1) return launch_small(e, publish_kernel_ptr(), base, -1);
+    const bool fstr = forced_h != nullptr && mode == kChain;
+    auto fb = [&](int k) { return forced_h ? int(forced_h[k]) : -1; };
+    bool decided = false;
+    for (int k = k0; k < k1; k++) {
+        if (!decided) {
+            const int nm = norm_for(e, mode, k, fb(k), fstr);
+            if (nm >= 0) {
+                Epi ep = base;
+                ep.k = k;
+                ep.role = kRoleNorm;
+                if (int rc = launch_sweep(e, k, nm, ep)) return rc;
+            } else {
+                if (int rc = launch_small(e, decide_kernel_ptr(), base, k)) return rc;
             }
         }
-        if (int rc = launch_decide(e, a, forced_d, U_d, bits_d, cond_d, out_d)) return rc;
-        if (k < k1) {
-            if (int rc = launch_sweep(e, k, kWrite)) return rc;
-            prev_grid = e->launch[size_t(k)].grid;
+        Epi ep = base;
+        ep.k = k;
+        ep.role = kRoleWrite;
+        int sweep = kWrite;
+        if (k + 1 == k1) {
+            ep.next = kNextPublish;
+        } else {
+            const int nn = norm_for(e, mode, k + 1, fb(k + 1), fstr);
+            const auto& L = e->launch[size_t(k)];
+            if (nn < 0) {
+                ep.next = kNextDecide;
+            } else if (e->allow_fusion && e->fusable[size_t(k)] && (nn == kGram ? L.wgram : L.wdirect)) {
+                ep.next = nn;
+                sweep = nn == kGram ? kWriteGram : kWriteDirect;
+            } else {
+                ep.next = kNextSweep;
+            }
         }
+        if (int rc = launch_sweep(e, k, sweep, ep)) return rc;
+        decided = ep.next != kNextSweep;
     }
     return 0;
 }
 
+// ---- GF(2) helpers over u64 vectors (host)
+
+// coef such that XOR of basis[i] over the bits i of coef == x; false if x is not in the span.
+bool gf2_solve(const std::vector<uint64_t>& basis, uint64_t x, uint64_t& coef) {
+    std::vector<std::pair<uint64_t, uint64_t>> rows;          // (vector, combination), distinct top bits
+    for (size_t i = 0; i < basis.size(); i++) {
+        uint64_t v = basis[i], c = uint64_t(1) << i;
+        for (const auto& r : rows)
+            if ((v >> (63 - __builtin_clzll(r.first))) & 1) {
+                v ^= r.first;
+                c ^= r.second;
+            }
+        if (!v) return false;                                     // basis not independent
+        rows.emplace_back(v, c);
+        std::sort(rows.begin(), rows.end(), [](const auto& p, const auto& q) { return p.first > q.first; });
+    }
+    uint64_t c = 0;
+    for (const auto& r : rows)
+        if ((x >> (63 - __builtin_clzll(r.first))) & 1) {
+            x ^= r.first;
+            c ^= r.second;
+        }
+    coef = c;
+    return x == 0;
+}
+
+int parity64(uint64_t x) { return __builtin_parityll(x); }
+
+void set_geometry(KStep& ks, int lv) {
+    const int lt = lv + 8;
+    if (ks.nk <= lt) {
+        ks.lg_grid = 0;
+        ks.lg_iter = 0;
+    } else {
+        const int tiles = ks.nk - lt;
+        ks.lg_grid = std::min(tiles, kMaxGridLog2);
+        ks.lg_iter = tiles - ks.lg_grid;
+    }
+}
+
+// Fused descriptor of step k (its write also reduces step k+1's norms), or false when step k+1 is not a
+// two-slot step or the pairing vector cannot become a thread coordinate.
+bool build_fused(const gs_engine* e, int k, KStep& f) {
+    const Plan& P = e->plan;
+    if (k + 1 >= P.ns) return false;
+    const StepInfo& s0 = P.steps[size_t(k)];
+    const KStep& p0 = e->hplain[size_t(k)];
+    const KStep& p1 = e->hplain[size_t(k + 1)];
+    if (p1.ntq != 2 || p1.growth) return false;
+    const int nk = s0.nk, lrf = fused_lr(e->storage);
+    for (int i = 0; i < lrf; i++)
+        if (i >= nk || s0.cols[size_t(i)] != (uint64_t(1) << i)) return false;
+    // live space after step k: kept columns, then g
+    std::vector<uint64_t> live(s0.cols.begin(), s0.cols.end());
+    if (s0.m == 2) live.push_back(s0.g);
+    const uint64_t tprime = p1.off[1];
+    uint64_t coef = 0;
+    if (!gf2_solve(live, tprime, coef)) return false;
+    const uint64_t cmask = nk >= 64 ? ~uint64_t(0) : (uint64_t(1) << nk) - 1;
+    const uint64_t gamma = coef & cmask;
+    const int eps = s0.m == 2 ? int((coef >> nk) & 1) : 0;
+    if ((gamma >> lrf) == 0) return false;
+    const int ptop = 63 - __builtin_clzll(gamma);
+    // coordinates: unit bits 0..lrf-1, gamma, then the remaining unit vectors except ptop
+    std::vector<uint64_t> basis;
+    for (int i = 0; i < lrf; i++) basis.push_back(uint64_t(1) << i);
+    basis.push_back(gamma);
+    for (int j = lrf; j < nk; j++)
+        if (j != ptop) basis.push_back(uint64_t(1) << j);
+    // slot index (phi) and sign (nu) of the next step as functionals of a physical address in the live space
+    const StepInfo& s1 = P.steps[size_t(k + 1)];
+    std::vector<uint64_t> nb(s1.cols.begin(), s1.cols.end());
+    nb.push_back(tprime);
+    auto phinu = [&](uint64_t x, int& phi, int& nu) {
+        uint64_t c2 = 0;
+        if (!gf2_solve(nb, x, c2)) return false;
+        phi = int((c2 >> s1.nk) & 1);
+        const uint64_t c1 = s1.nk >= 64 ? c2 : c2 & ((uint64_t(1) << s1.nk) - 1);
+        nu = parity64(c1 & s1.rhoc);
+        return true;
+    };
+    std::vector<int> phic(static_cast<size_t>(nk)), nuc(static_cast<size_t>(nk));
+    for (int j = 0; j < nk; j++)
+        if (!phinu(s0.cols[size_t(j)], phic[size_t(j)], nuc[size_t(j)])) return false;
+    int phig = 0, nug = 0;
+    if (s0.m == 2 && !phinu(s0.g, phig, nug)) return false;
+    f = p0;
+    memset(f.cols, 0, sizeof f.cols);
+    f.pmask[0] = f.pmask[1] = f.pmask[2] = 0;
+    for (int i = 0; i < nk; i++) {
+        const uint64_t b = basis[size_t(i)];
+        uint64_t col = 0;
+        int pr = 0, pp = 0, pn = 0;
+        for (int j = 0; j < nk; j++)
+            if ((b >> j) & 1) {
+                col ^= s0.cols[size_t(j)];
+                pp ^= phic[size_t(j)];
+                pn ^= nuc[size_t(j)];
+            }
+        pr = parity64(b & s0.rhoc);
+        f.cols[i] = col;
+        f.pmask[0] |= uint64_t(pr) << i;
+        f.pmask[1] |= uint64_t(pp) << i;
+        f.pmask[2] |= uint64_t(pn) << i;
+    }
+    f.lr = lrf;
+    f.vg = 2;
+    f.route2 = p1.route;
+    f.par2 = int((p1.parmask >> 1) & 1);
+    f.eps = eps;
+    f.phig = phig;
+    f.nug = nug;
+    set_geometry(f, lrf + 1);
+    return true;
+}
+
 int build_steps(gs_engine* e) {
     const Plan& P = e->plan;
-    e->hsteps.assign(size_t(P.ns), KStep{});
+    e->hplain.assign(size_t(P.ns), KStep{});
+    e->hfused.assign(size_t(P.ns), KStep{});
+    e->fusable.assign(size_t(P.ns), 0);
     e->launch.assign(size_t(P.ns), gs_engine::Launch{});
-    const int vmax = preferred_v(e->storage);
+    const int lrmax = plain_lr(e->storage);
     for (int k = 0; k < P.ns; k++) {
         const StepInfo& si = P.steps[size_t(k)];
-        KStep& ks = e->hsteps[size_t(k)];
+        KStep& ks = e->hplain[size_t(k)];
         memset(&ks, 0, sizeof ks);
         for (int j = 0; j < si.nk; j++) ks.cols[j] = si.cols[size_t(j)];
         ks.g = si.g;
-        ks.rhoc = si.rhoc;
+        ks.pmask[0] = si.rhoc;
         for (int b = 0; b < 2; b++)
             for (int x = 0; x < 2; x++) {
                 ks.mre[b][x] = si.M[b][x].real();
@@ -506,27 +529,31 @@ int build_steps(gs_engine* e) {
         ks.ntq = si.ntq;
         ks.m = si.m;
         ks.growth = si.growth ? 1 : 0;
+        ks.vg = 1;
         std::string err;
         if (int rc = route_and_relabel(si, ks, err)) return fail(rc, err + " at step " + std::to_string(k));
-        // cosets per thread: 2^vmax when the low vmax kept columns are the unit bits 0..vmax-1
+        // a run of 2^lrmax cosets per thread when the low kept columns are the unit bits 0..lrmax-1
         int unit = 0;
-        while (unit < si.nk && unit < vmax && si.cols[size_t(unit)] == (uint64_t(1) << unit)) unit++;
-        ks.v = unit == vmax ? vmax : 0;
-        const int lt = ks.v + 8;
-        if (si.nk <= lt) {
-            ks.lg_grid = 0;
-            ks.lg_iter = 0;
-        } else {
-            const int tiles = si.nk - lt;
-            ks.lg_grid = std::min(tiles, kMaxGridLog2);
-            ks.lg_iter = tiles - ks.lg_grid;
-        }
+        while (unit < si.nk && unit < lrmax && si.cols[size_t(unit)] == (uint64_t(1) << unit)) unit++;
+        ks.lr = unit == lrmax ? lrmax : 0;
+        set_geometry(ks, ks.lr);
         auto& L = e->launch[size_t(k)];
         L.grid = 1 << ks.lg_grid;
-        L.gram = sweep_kernel_for(e->storage, ks.v, ks.ntq, ks.route, kGram);
-        L.direct = sweep_kernel_for(e->storage, ks.v, ks.ntq, ks.route, kDirect);
-        L.write = sweep_kernel_for(e->storage, ks.v, ks.ntq, ks.route, kWrite);
+        L.gram = sweep_kernel_for(e->storage, ks.lr, 1, ks.ntq, ks.route, kGram, 0);
+        L.direct = sweep_kernel_for(e->storage, ks.lr, 1, ks.ntq, ks.route, kDirect, 0);
+        L.write = sweep_kernel_for(e->storage, ks.lr, 1, ks.ntq, ks.route, kWrite, 0);
         if (!L.gram || !L.direct || !L.write) return fail(15, "no kernel for step " + std::to_string(k));
+    }
+    for (int k = 0; k + 1 < P.ns; k++) {
+        KStep f;
+        if (!build_fused(e, k, f)) continue;
+        auto& L = e->launch[size_t(k)];
+        L.wgram = sweep_kernel_for(e->storage, f.lr, 2, f.ntq, f.route, kWriteGram, f.route2);
+        L.wdirect = sweep_kernel_for(e->storage, f.lr, 2, f.ntq, f.route, kWriteDirect, f.route2);
+        if (!L.wgram || !L.wdirect) continue;
+        L.fgrid = 1 << f.lg_grid;
+        e->hfused[size_t(k)] = f;
+        e->fusable[size_t(k)] = 1;
     }
     return 0;
 }
@@ -536,9 +563,10 @@ void free_engine(gs_engine* e) {
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     (void)hipFree(e->arena);
-    (void)hipFree(e->dsteps);
-    (void)hipFree(e->part_n);
-    (void)hipFree(e->part_w);
+    (void)hipFree(e->dplain);
+    (void)hipFree(e->dfused);
+    (void)hipFree(e->part);
+    (void)hipFree(e->counter);
     (void)hipFree(e->ctl);
     (void)hipFree(e->d_forced);
     (void)hipFree(e->d_U);
@@ -707,9 +735,9 @@ int gs_engine_create(const gs_plan* plan, int storage, int device, gs_engine** o
     auto host_alloc = [&](void** p, size_t n) { return hipHostMalloc(p, std::max<size_t>(n, 16), 0) == hipSuccess; };
     const size_t C = gs_engine::kChunk;
     const size_t nsz = size_t(std::max(ns, 1));
-    bool ok = dev_alloc((void**)&e->dsteps, sizeof(KStep) * nsz) &&
-              dev_alloc((void**)&e->part_n, sizeof(double) * kPartStride << kMaxGridLog2) &&
-              dev_alloc((void**)&e->part_w, sizeof(double) * kPartStride << kMaxGridLog2) &&
+    bool ok = dev_alloc((void**)&e->dplain, sizeof(KStep) * nsz) && dev_alloc((void**)&e->dfused, sizeof(KStep) * nsz) &&
+              dev_alloc((void**)&e->part, sizeof(double) * kPartStride << kMaxGridLog2) &&
+              dev_alloc((void**)&e->counter, sizeof(unsigned)) &&
               dev_alloc((void**)&e->ctl, sizeof(Ctl)) && dev_alloc((void**)&e->d_forced, C * nsz) &&
               dev_alloc((void**)&e->d_U, sizeof(double) * C * nsz) && dev_alloc((void**)&e->d_bits, C * nsz) &&
               dev_alloc((void**)&e->d_cond, sizeof(double) * C * nsz) &&
@@ -721,10 +749,13 @@ int gs_engine_create(const gs_plan* plan, int storage, int device, gs_engine** o
     if (hipStreamCreate(&e->stream) != hipSuccess || hipEventCreate(&e->ev0) != hipSuccess ||
         hipEventCreate(&e->ev1) != hipSuccess)
         return bail(fail(17, "stream/event creation failed"));
-    if (ns > 0 && hipMemcpy(e->dsteps, e->hsteps.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess)
+    if (ns > 0 && (hipMemcpy(e->dplain, e->hplain.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess ||
+                   hipMemcpy(e->dfused, e->hfused.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess))
         return bail(fail(17, "plan upload failed"));
-    e->prof_ms.assign(nsz * 3, 0.0);
-    e->prof_n.assign(nsz * 3, 0);
+    if (hipMemset(e->counter, 0, sizeof(unsigned)) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return bail(fail(17, "counter initialisation failed"));
+    e->prof_ms.assign(nsz * kNumSweeps, 0.0);
+    e->prof_n.assign(nsz * kNumSweeps, 0);
     *out = e;
     return 0;
 }
@@ -853,7 +884,7 @@ int gs_prefix(gs_engine* eng, const int8_t* bits, int K, double* table_reim, int
         return fail(11, "allocation of the dump buffer failed");
     }
     HIP_TRY(hipMemcpy(doffs, offs.data(), sizeof(uint64_t) * offs.size(), hipMemcpyHostToDevice));
-    const KStep* sk = K < ns ? e->dsteps + K : nullptr;
+    const KStep* sk = K < ns ? e->dplain + K : nullptr;
     const int grid = int(std::min<int64_t>((n + kThreads - 1) / kThreads, 65536));
     if (e->storage == GS_C64)
         hipLaunchKernelGGL(gather_kernel<float2>, dim3(grid), dim3(kThreads), 0, e->stream, sk,
@@ -937,16 +968,16 @@ int gs_resume(gs_engine* eng, const int8_t* bits, int K, int nslices, const doub
                                cshift ? cshift[s] : 0ull, static_cast<const int*>(dperm) + size_t(s) * ntq, ntq, n);
         }
         if (rc) break;
-        const KStep* sk = K < ns ? e->dsteps + K : nullptr;
+        const KStep* sk = K < ns ? e->dplain + K : nullptr;
         if (e->storage == GS_C64)
             hipLaunchKernelGGL(scatter_kernel<float2>, dim3(grid), dim3(kThreads), 0, e->stream, sk,
                                static_cast<float2*>(e->arena), static_cast<const uint64_t*>(doffs), ntq, n,
-                               static_cast<const double2*>(dacc), e->part_w);
+                               static_cast<const double2*>(dacc), e->part);
         else
             hipLaunchKernelGGL(scatter_kernel<double2>, dim3(grid), dim3(kThreads), 0, e->stream, sk,
                                static_cast<double2*>(e->arena), static_cast<const uint64_t*>(doffs), ntq, n,
-                               static_cast<const double2*>(dacc), e->part_w);
-        hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(kThreads), 0, e->stream, e->part_w, grid, e->d_out + 4);
+                               static_cast<const double2*>(dacc), e->part);
+        hipLaunchKernelGGL(sum_parts_kernel, dim3(1), dim3(1), 0, e->stream, e->part, grid, e->d_out + 4);
         double norm = 0;
         if (hipMemcpyAsync(&e->h_out[4], e->d_out + 4, sizeof(double), hipMemcpyDeviceToHost, e->stream) != hipSuccess ||
             hipStreamSynchronize(e->stream) != hipSuccess) {
@@ -1039,15 +1070,27 @@ int gs_profile_read(const gs_engine* eng, double* ms, int64_t* launches) {
     return 0;
 }
 
-// Launch geometry of a step: cosets per thread (log2), blocks, tiles per block (log2), offsets, route.
-int gs_step_geometry(const gs_engine* eng, int k, int* v, int* grid, int* lg_iter, int* ntq, int* route) {
-    if (!eng || k < 0 || k >= eng->plan.ns) return fail(16, "bad step");
-    const KStep& s = eng->hsteps[size_t(k)];
-    if (v) *v = s.v;
-    if (grid) *grid = 1 << s.lg_grid;
-    if (lg_iter) *lg_iter = s.lg_iter;
-    if (ntq) *ntq = s.ntq;
-    if (route) *route = s.route;
+// Launch geometry of step k, out[8]: plain log2 run, plain blocks, ntq, route, fusable, fused blocks, next route,
+// plain log2 tiles per block.
+int gs_step_geometry(const gs_engine* eng, int k, int* out) {
+    if (!eng || k < 0 || k >= eng->plan.ns || !out) return fail(16, "bad step");
+    const KStep& s = eng->hplain[size_t(k)];
+    const KStep& f = eng->hfused[size_t(k)];
+    out[0] = s.lr;
+    out[1] = 1 << s.lg_grid;
+    out[2] = s.ntq;
+    out[3] = s.route;
+    out[4] = eng->fusable[size_t(k)];
+    out[5] = eng->fusable[size_t(k)] ? 1 << f.lg_grid : 0;
+    out[6] = eng->fusable[size_t(k)] ? f.route2 : 0;
+    out[7] = s.lg_iter;
+    return 0;
+}
+
+// Turn the folding of the next step's norms into the previous write off or on (for measurements; default on).
+int gs_set_fusion(gs_engine* eng, int on) {
+    if (!eng) return fail(16, "null engine");
+    eng->allow_fusion = on != 0;
     return 0;
 }
 

@@ -1,31 +1,38 @@
-// Sweep kernels of the in-place cut-rank dynamic program on CDNA4 (gfx950).
+// Sweep kernels and on-device chain logic of the in-place cut-rank dynamic program on CDNA4 (gfx950).
 //
 // One step reads, for every coset representative c < 2^nk, the ntq live slots base(c) ^ off_j (base is GF(2)-linear
-// in c: XOR of the kept columns selected by the bits of c) and produces for each measured bit b and output tau
+// in c: XOR of the coordinate columns selected by the bits of c) and produces for each measured bit b and output tau
 //   out[b][tau] = sum_{terms} M[b][x] (-1)^{x (parity(c & rhoc) ^ par_j)} mem[base(c) ^ off_j]
-// (gsamp2.cpp:32-38).  The sums are regrouped as out[b][tau] = M[b][0] A[tau][0] + M[b][1] A[tau][1] with
-// A[tau][x] the signed slot sums routed to (tau, x); all arithmetic is FP64, the arena is complex<float> (c64) or
+// (gsamp2.cpp:32-38).  The sums are regrouped as out[b][tau] = M[b][0] A[tau][0] + M[b][1] A[tau][1] with A[tau][x]
+// the signed slot sums routed to (tau, x); all arithmetic is FP64, the arena is complex<float> (c64) or
 // complex<double> (c128), and a store rounds once (gsamp2.cpp:35, :52).
 //
-// Three sweeps share the addressing:
-//   kGram   -> per block, the 2x2 Gram matrix G[x][x'] = sum_c sum_tau A[tau][x] conj(A[tau][x']); the branch norms
-//              n_b = sum_{x,x'} M[b][x] conj(M[b][x']) G[x][x'] are formed by the decide kernel (sampling steps);
-//   kDirect -> per block, n_0 and n_1 summed directly as |out[b][tau]|^2 (the reference's norm pass, used where a
-//              branch with a tiny probability must keep full relative precision: forced strings);
-//   kWrite  -> stores out[bv][tau] * sc in place (tau 0 at base, tau 1 at base ^ g) and returns, per block, the FP64
-//              norm of the written values and the norm of the rounded stored values.
+// Sweeps (template SWEEP):
+//   kGram        per block, the 2x2 Gram matrix G[x][x'] = sum_c sum_tau A[tau][x] conj(A[tau][x']); the branch norms
+//                n_b = sum_{x,x'} M[b][x] conj(M[b][x']) G[x][x'] follow without recomputing both branches;
+//   kDirect      per block, n_0 and n_1 summed directly as |out[b][tau]|^2 (full relative precision for forced
+//                strings, the reference's norm pass);
+//   kWrite       stores out[bv][tau] * sc in place (tau 0 at base, tau 1 at base ^ g) and returns the FP64 norm of the
+//                written values and of the rounded stored values;
+//   kWriteGram / kWriteDirect  kWrite, plus the Gram matrix / direct norms of the NEXT step computed from the values
+//                just stored: the thread's cosets are chosen to span the next step's pairing vector, so both slots of
+//                each next-step coset are in the thread's registers.  This removes the next step's norm pass (one
+//                read of the table per sampled step).
 //
-// Layout and addressing.  A block of 256 threads handles a tile of 256 * V consecutive cosets per iteration; thread t
-// owns cosets tile*256V + t*V + i, i < V.  The low v = log2 V kept columns are the unit bits 0..v-1 (the host only
-// picks V > 1 when they are), so a thread's V cosets have bases ag ^ i for one ag, and the slot group
-// {ag ^ off_j ^ i : i < V} is one aligned run of V entries: one 16-byte (or two) vector load per slot, coalesced
-// across the wave whenever consecutive threads differ only in low physical bits.  XOR with the low bits of ag ^ off_j
-// permutes the run; the permutation is undone in registers.  Each thread writes exactly the slots it read, so the
-// sweep is race free in place (the planner's invariant: a coset's outputs land in its own slots, or in fresh slots
-// for growth steps).
+// Layout.  A block of 256 threads handles a tile of 256 * V cosets per iteration; thread t owns the V = VR * VG
+// cosets whose low coordinates run over 0..V-1.  Coordinates 0..log2(VR)-1 have unit physical columns, so the VR
+// cosets' slots are one aligned run (a 16-byte vector access); with VG = 2, coordinate log2(VR) has an arbitrary
+// column Gamma (a second run).  XOR with low offset bits permutes a run; the permutation is undone in registers.
+// Each thread writes exactly the slots it read (or fresh slots of a growth step): race free in place.
 //
-// Tiles are enumerated in Gray-code order inside a block, so moving to the next tile XORs one kept column into the
-// base and flips at most one sign bit: address generation costs O(1) scalar work per tile.
+// Tiles are enumerated in Gray-code order inside a block, so the next tile XORs one column into the base address and
+// flips the parity bits by one mask: O(1) scalar work per tile.
+//
+// Chain logic.  The last block of every sweep (detected with an agent-scope atomic counter after each block has
+// published its partial sums) reduces the partials in a fixed order and runs the chain on the device: the bookkeeping
+// of a write (chain1 / run_steps conditionals, the stored table's norm) and the decision of the next step (forced, or
+// U_k < p1 with U_k drawn on the host: gsamp2.cpp:47, :86, :108 draw one uniform per sampled step from a stream that
+// does not depend on p1).  No host round trip, no separate reduction launch.
 #include "gs_device.h"
 
 namespace gs {
@@ -48,8 +55,8 @@ struct Cx<double2> {
     static __device__ __forceinline__ double2 narrow(double2 a) { return a; }
 };
 
-// V consecutive arena entries at an aligned index, as 16-byte vector accesses.
-template <class S, int V>
+// VR consecutive arena entries at an aligned index, as 16-byte vector accesses.
+template <class S, int VR>
 struct Run;
 
 template <>
@@ -100,22 +107,22 @@ struct Run<double2, 2> {
     }
 };
 
-// e[i] <- e[i ^ p] for p < V (per-lane conditional swaps; an involution, used for loads and stores alike).
-template <class S, int V>
-__device__ __forceinline__ void permute(S (&e)[V], unsigned p) {
-    if constexpr (V >= 2) {
+// e[i] <- e[i ^ p] for p < VR (per-lane conditional swaps; an involution, used for loads and stores alike).
+template <class S, int VR>
+__device__ __forceinline__ void permute(S (&e)[VR], unsigned p) {
+    if constexpr (VR >= 2) {
         const bool s1 = p & 1;
 #pragma unroll
-        for (int i = 0; i < V; i += 2) {
+        for (int i = 0; i < VR; i += 2) {
             const S a = e[i], b = e[i + 1];
             e[i] = s1 ? b : a;
             e[i + 1] = s1 ? a : b;
         }
     }
-    if constexpr (V >= 4) {
+    if constexpr (VR >= 4) {
         const bool s2 = p & 2;
 #pragma unroll
-        for (int i = 0; i < V; i++) {
+        for (int i = 0; i < VR; i++) {
             if (i & 2) continue;
             const S a = e[i], b = e[i + 2];
             e[i] = s2 ? b : a;
@@ -129,9 +136,7 @@ __device__ __forceinline__ double2 negate_if(double2 a, int s) {
     return make_double2(__longlong_as_double(__double_as_longlong(a.x) ^ m),
                         __longlong_as_double(__double_as_longlong(a.y) ^ m));
 }
-
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-
 // a * b + c (complex, FP64)
 __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
     return make_double2(fma(a.x, b.x, fma(-a.y, b.y, c.x)), fma(a.x, b.y, fma(a.y, b.x, c.y)));
@@ -151,41 +156,222 @@ __device__ __host__ constexpr int tau_x1(int j) {
     return ROUTE == kRoute1 ? 0 : ROUTE == kRouteSplit ? 1 : ROUTE == kRouteMixA ? 1 - (j & 1) : (j & 1);
 }
 
-// Sum over the wave, then over the block's 4 waves in a fixed order; thread 0 writes NA doubles.
-template <int NA>
-__device__ __forceinline__ void block_sum_store(double (&acc)[NA], double* out) {
+// Accumulate the routed signed slot sums of one coset: A[tau][x] += slot values (sign (-1)^{pc ^ par_j} on x = 1).
+template <int ROUTE, int NTQ, int MO>
+__device__ __forceinline__ void route_terms(double2 (&A)[MO][2], const double2 (&w)[NTQ], int pc, uint32_t parmask) {
 #pragma unroll
-    for (int q = 0; q < NA; q++)
-        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
-    __shared__ double sh[kThreads / 64][NA];
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0)
+    for (int t = 0; t < MO; t++) A[t][0] = A[t][1] = make_double2(0, 0);
 #pragma unroll
-        for (int q = 0; q < NA; q++) sh[wave][q] = acc[q];
-    __syncthreads();
-    if (threadIdx.x == 0)
-#pragma unroll
-        for (int q = 0; q < NA; q++) {
-            double t = sh[0][q];
-#pragma unroll
-            for (int w = 1; w < kThreads / 64; w++) t += sh[w][q];
-            out[q] = t;
-        }
+    for (int j = 0; j < NTQ; j++) {
+        A[tau_x0<ROUTE>(j)][0] = cadd(A[tau_x0<ROUTE>(j)][0], w[j]);
+        A[tau_x1<ROUTE>(j)][1] = cadd(A[tau_x1<ROUTE>(j)][1], negate_if(w[j], pc ^ int(parmask >> j)));
+    }
 }
 
-template <class S, int V, int NTQ, int ROUTE, int SWEEP>
+template <int MO>
+__device__ __forceinline__ void gram_acc(const double2 (&A)[MO][2], double* acc) {
+#pragma unroll
+    for (int t = 0; t < MO; t++) {
+        const double2 a = A[t][0], b = A[t][1];
+        acc[0] += norm2(a);
+        acc[1] += norm2(b);
+        acc[2] = fma(a.x, b.x, fma(a.y, b.y, acc[2]));      // Re a conj(b)
+        acc[3] = fma(a.y, b.x, fma(-a.x, b.y, acc[3]));     // Im a conj(b)
+    }
+}
+
+template <int MO>
+__device__ __forceinline__ void direct_acc(const double2 (&A)[MO][2], const double2 (&M)[2][2], double* acc) {
+#pragma unroll
+    for (int t = 0; t < MO; t++) {
+        acc[0] += norm2(cfma(M[0][0], A[t][0], cmul(M[0][1], A[t][1])));
+        acc[1] += norm2(cfma(M[1][0], A[t][0], cmul(M[1][1], A[t][1])));
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// chain logic (one thread)
+
+__device__ __forceinline__ double2 step_m(const KStep& s, int b, int x) {
+    return make_double2(s.mre[b][x], s.mim[b][x]);
+}
+
+__device__ void publish(Ctl& c, const Epi& e) {
+    if (e.mode == kChain1 && !c.dead) c.logp = log2(c.prev) - e.ns;
+    e.out[0] = c.logp;
+    e.out[1] = c.logscale;
+    e.out[2] = c.prev;
+    e.out[3] = c.dead;
+}
+
+// Bookkeeping of the write of step k (norm nb of the written FP64 values, nst of the rounded stored values).
+__device__ void finish_write(Ctl& c, const Epi& e, int k, double nb, double nst) {
+    c.nstored = nst;
+    if (e.mode == kChain1) {                                   // gsamp2.cpp:67
+        e.cond[k] = nb / (2 * c.prev);
+        c.prev = nb;
+        if (nb <= 0) {
+            for (int q = k; q < e.ns; q++) e.cond[q] = 0;
+            c.logp = -INFINITY;
+            c.dead = 1;
+        }
+    } else if (e.mode == kRun) {                               // gsamp2.cpp:93-95
+        c.logscale += 0.5 * log2(c.prev);
+        e.cond[k] = nb / 2;
+        c.prev = nb;
+        if (nb <= 0) {
+            for (int q = k; q < e.k1; q++) {
+                e.cond[q] = 0;
+                e.bits[q] = e.forced[q] < 0 ? 0 : e.forced[q];
+            }
+            c.logp = -INFINITY;
+            c.dead = 1;
+        } else {
+            c.logp += log2(e.cond[k]);
+        }
+    }
+    if (c.dead) publish(c, e);
+}
+
+// Decision of step k from its norms: norm == kGram (red = G00, G11, Re G01, Im G01), kDirect (red = n0, n1), or -1
+// (no norms: forced step, chain1, or a growth step whose norms follow from the stored table's norm).
+__device__ void decide(Ctl& c, const Epi& e, int k, int norm, const double* red) {
+    const KStep& s = e.plain[k];
+    int bv = e.forced ? e.forced[k] : -1;
+    double sc = 1.0;
+    const bool need_p = (e.mode == kChain) || (e.mode == kRun && bv < 0);
+    double n0 = 0, n1 = 0;
+    if (need_p) {
+        for (int b = 0; b < 2; b++) {
+            const double2 m0 = step_m(s, b, 0), m1 = step_m(s, b, 1);
+            double nbv;
+            if (norm == kGram) {
+                const double pr = m0.x * m1.x + m0.y * m1.y, pi = m0.y * m1.x - m0.x * m1.y;   // m0 conj(m1)
+                nbv = norm2(m0) * red[0] + norm2(m1) * red[1] + 2 * (pr * red[2] - pi * red[3]);
+            } else if (norm == kDirect) {
+                nbv = red[b];
+            } else {
+                nbv = (norm2(m0) + norm2(m1)) * c.nstored;
+            }
+            (b ? n1 : n0) = nbv;
+        }
+    }
+    if (e.mode == kChain) {                                    // gsamp2.cpp:46-49
+        const double p1 = n1 / (n0 + n1);
+        if (bv < 0) bv = e.U[k] < p1;
+        const double pb = bv ? p1 : 1 - p1;
+        if (pb <= 0) {
+            c.logp = -INFINITY;
+            c.dead = 1;
+            publish(c, e);
+            return;
+        }
+        c.logp += log2(pb);
+        sc = 1.0 / sqrt(bv ? n1 : n0);
+    } else if (e.mode == kChain1) {
+        sc = 1.0;
+    } else {                                                   // gsamp2.cpp:80-87
+        sc = 1.0 / sqrt(c.prev);
+        if (bv < 0) bv = e.U[c.qidx++] < n1 / (n0 + n1);
+    }
+    e.bits[k] = int8_t(bv);
+    c.bv = bv;
+    c.sc = sc;
+    const double2 m0 = step_m(s, bv, 0), m1 = step_m(s, bv, 1);
+    c.w[0] = m0.x * sc;
+    c.w[1] = m0.y * sc;
+    c.w[2] = m1.x * sc;
+    c.w[3] = m1.y * sc;
+}
+
+// Last block of a sweep: reduce the nq partial sums of all blocks in a fixed order, then run the chain.
+template <int NQ>
+__device__ void sweep_epilogue(double (&acc)[NQ], double* part, Ctl* ctl, const Epi& e, int sweep) {
+    // block sum (wave butterfly, then the 4 waves in order)
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
+    __shared__ double sh[kThreads][NQ];
+    __shared__ int s_last;
+    const int t = threadIdx.x;
+    if ((t & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) sh[t >> 6][q] = acc[q];
+    __syncthreads();
+    if (t == 0) {
+        double* mine = part + size_t(blockIdx.x) * kPartStride;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            double v = sh[0][q];
+#pragma unroll
+            for (int w = 1; w < kThreads / 64; w++) v += sh[w][q];
+            __hip_atomic_store(mine + q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const unsigned done = __hip_atomic_fetch_add(e.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = done == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    double r[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) r[q] = 0;
+    for (unsigned i = t; i < gridDim.x; i += kThreads)
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            r[q] += __hip_atomic_load(part + size_t(i) * kPartStride + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) sh[t][q] = r[q];
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (t < w)
+#pragma unroll
+            for (int q = 0; q < NQ; q++) sh[t][q] += sh[t + w][q];
+        __syncthreads();
+    }
+    if (t != 0) return;
+    double red[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) red[q] = sh[0][q];
+    Ctl& c = *ctl;
+    if (e.role == kRoleNorm) {
+        decide(c, e, e.k, sweep, red);
+    } else {
+        finish_write(c, e, e.k, red[0], red[1]);
+        if (!c.dead) {
+            if (e.k + 1 == e.k1) publish(c, e);
+            else if (e.next == kNextDecide) decide(c, e, e.k + 1, -1, nullptr);
+            else if (e.next == kNextFusedGram) decide(c, e, e.k + 1, kGram, red + 2);
+            else if (e.next == kNextFusedDirect) decide(c, e, e.k + 1, kDirect, red + 2);
+        }
+    }
+    __hip_atomic_store(e.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// the sweep
+
+template <class S, int VR, int VG, int NTQ, int ROUTE, int SWEEP, int ROUTE2>
 __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict__ sp, S* __restrict__ mem,
-                                                         double* __restrict__ part, const Ctl* __restrict__ ctl) {
+                                                         double* __restrict__ part, Ctl* __restrict__ ctl, Epi epi) {
     constexpr int MO = ROUTE == kRoute1 ? 1 : 2;
-    constexpr int LV = V == 1 ? 0 : V == 2 ? 1 : 2;
-    constexpr int NA = SWEEP == kGram ? 4 : 2;
+    constexpr int MO2 = ROUTE2 == kRoute1 ? 1 : 2;
+    constexpr int LR = VR == 1 ? 0 : VR == 2 ? 1 : 2;
+    constexpr int LV = LR + (VG == 2 ? 1 : 0);
+    constexpr int V = VR * VG;
+    constexpr bool WRITES = SWEEP >= kWrite;
+    constexpr bool FUSED = SWEEP == kWriteGram || SWEEP == kWriteDirect;
+    constexpr int NA = SWEEP == kGram ? 4 : SWEEP == kDirect ? 2 : SWEEP == kWrite ? 2 : SWEEP == kWriteGram ? 6 : 4;
     if (ctl->dead) return;
     const KStep& s = *sp;
     const int nk = s.nk;
     const int tid = threadIdx.x;
-    const int lt = LV + 8;                         // log2 cosets per tile
+    constexpr int lt = LV + 8;                     // log2 cosets per tile
 
-    // thread part of the base address and of the sign parity (bits LV .. LV+7 of c)
+    // parity bits of a coordinate: bit 0 output sign, bit 1 next-step slot index, bit 2 next-step sign
+    auto pb = [&](int j) -> int {
+        return int((s.pmask[0] >> j) & 1) | (int((s.pmask[1] >> j) & 1) << 1) | (int((s.pmask[2] >> j) & 1) << 2);
+    };
     uint64_t a_thr = 0;
     int p_thr = 0;
 #pragma unroll
@@ -193,38 +379,50 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
         const int j = LV + b;
         if (j < nk && ((tid >> b) & 1)) {
             a_thr ^= s.cols[j];
-            p_thr ^= int((s.rhoc >> j) & 1);
+            p_thr ^= pb(j);
         }
     }
     const bool active = nk >= lt || (uint64_t(tid) << LV) < (uint64_t(1) << nk);
-    // block part (bits lt .. lt+lg_grid-1)
     uint64_t a_blk = 0;
     int p_blk = 0;
     for (int b = 0; b < s.lg_grid; b++)
         if ((blockIdx.x >> b) & 1) {
             a_blk ^= s.cols[lt + b];
-            p_blk ^= int((s.rhoc >> (lt + b)) & 1);
+            p_blk ^= pb(lt + b);
         }
-    const int jit = lt + s.lg_grid;                // first column of the iteration index
+    const int jit = lt + s.lg_grid;
     const int niter = 1 << s.lg_iter;
+    int plow[V];                                   // parity bits of the thread's low coordinates
+#pragma unroll
+    for (int i = 0; i < V; i++) {
+        int p = 0;
+#pragma unroll
+        for (int b = 0; b < LV; b++)
+            if ((i >> b) & 1) p ^= pb(b);
+        plow[i] = p;
+    }
+    const uint64_t gam = VG == 2 ? s.cols[LR] : 0;
 
-    double2 w0 = make_double2(0, 0), w1 = make_double2(0, 0);
-    double2 m00 = w0, m01 = w0, m10 = w0, m11 = w0;
-    if constexpr (SWEEP == kWrite) {
+    double2 w0 = make_double2(0, 0), w1 = w0;
+    double2 M[2][2] = {{w0, w0}, {w0, w0}};
+    if constexpr (WRITES) {
         w0 = make_double2(ctl->w[0], ctl->w[1]);
         w1 = make_double2(ctl->w[2], ctl->w[3]);
     }
-    if constexpr (SWEEP == kDirect) {
-        m00 = make_double2(s.mre[0][0], s.mim[0][0]);
-        m01 = make_double2(s.mre[0][1], s.mim[0][1]);
-        m10 = make_double2(s.mre[1][0], s.mim[1][0]);
-        m11 = make_double2(s.mre[1][1], s.mim[1][1]);
+    if constexpr (SWEEP == kDirect || SWEEP == kWriteDirect) {
+        const KStep& ms = SWEEP == kDirect ? epi.plain[epi.k] : epi.plain[epi.k + 1];
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int x = 0; x < 2; x++) M[b][x] = make_double2(ms.mre[b][x], ms.mim[b][x]);
     }
     uint64_t off[NTQ];
 #pragma unroll
     for (int j = 0; j < NTQ; j++) off[j] = s.off[j];
     const uint32_t parmask = s.parmask;
     const uint64_t g = s.g;
+    const int eps = s.eps, phig = s.phig, nug = s.nug;
+    const uint32_t par2mask = uint32_t(s.par2) << 1;
 
     double acc[NA];
 #pragma unroll
@@ -233,143 +431,188 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     uint64_t a_it = 0;
     int p_it = 0;
     for (int it = 0; it < niter; it++) {
-        if (it) {                                   // Gray code: one column changes
+        if (it) {                                   // Gray code: one coordinate changes
             const int b = __builtin_ctz(unsigned(it));
             a_it ^= s.cols[jit + b];
-            p_it ^= int((s.rhoc >> (jit + b)) & 1);
+            p_it ^= pb(jit + b);
         }
         if (!active) continue;
-        const uint64_t ag = a_thr ^ a_blk ^ a_it;   // coset i of this thread has base ag ^ i
+        const uint64_t ag = a_thr ^ a_blk ^ a_it;   // coset (ir, ig) of this thread has base ag ^ ir ^ ig * gam
         const int pbase = p_thr ^ p_blk ^ p_it;
-        S e[NTQ][V];
+        S e[NTQ][VG][VR];
 #pragma unroll
-        for (int j = 0; j < NTQ; j++) {
-            const uint64_t a = ag ^ off[j];
-            Run<S, V>::load(mem, a & ~uint64_t(V - 1), e[j]);
-            permute<S, V>(e[j], unsigned(a) & (V - 1));
-        }
-        S o0[V], o1[V];
+        for (int j = 0; j < NTQ; j++)
 #pragma unroll
-        for (int i = 0; i < V; i++) {
-            const int pc = pbase ^ __builtin_parity(unsigned(i) & unsigned(s.rhoc));
-            double2 A[MO][2];
-#pragma unroll
-            for (int t = 0; t < MO; t++) A[t][0] = A[t][1] = make_double2(0, 0);
-#pragma unroll
-            for (int j = 0; j < NTQ; j++) {
-                const double2 wv = Cx<S>::widen(e[j][i]);
-                A[tau_x0<ROUTE>(j)][0] = cadd(A[tau_x0<ROUTE>(j)][0], wv);
-                A[tau_x1<ROUTE>(j)][1] = cadd(A[tau_x1<ROUTE>(j)][1], negate_if(wv, pc ^ int(parmask >> j)));
+            for (int ig = 0; ig < VG; ig++) {
+                const uint64_t a = ag ^ off[j] ^ (ig ? gam : 0);
+                Run<S, VR>::load(mem, a & ~uint64_t(VR - 1), e[j][ig]);
+                permute<S, VR>(e[j][ig], unsigned(a) & (VR - 1));
             }
-            if constexpr (SWEEP == kGram) {
+        S o[2][VG][VR];
 #pragma unroll
-                for (int t = 0; t < MO; t++) {
-                    const double2 a = A[t][0], b = A[t][1];
-                    acc[0] += norm2(a);
-                    acc[1] += norm2(b);
-                    acc[2] = fma(a.x, b.x, fma(a.y, b.y, acc[2]));      // Re a conj(b)
-                    acc[3] = fma(a.y, b.x, fma(-a.x, b.y, acc[3]));     // Im a conj(b)
-                }
-            } else if constexpr (SWEEP == kDirect) {
+        for (int ig = 0; ig < VG; ig++)
 #pragma unroll
-                for (int t = 0; t < MO; t++) {
-                    acc[0] += norm2(cfma(m00, A[t][0], cmul(m01, A[t][1])));
-                    acc[1] += norm2(cfma(m10, A[t][0], cmul(m11, A[t][1])));
+            for (int ir = 0; ir < VR; ir++) {
+                const int i = ir + VR * ig;
+                const int pc = (pbase ^ plow[i]) & 1;
+                double2 wv[NTQ];
+#pragma unroll
+                for (int j = 0; j < NTQ; j++) wv[j] = Cx<S>::widen(e[j][ig][ir]);
+                double2 A[MO][2];
+                route_terms<ROUTE, NTQ, MO>(A, wv, pc, parmask);
+                if constexpr (SWEEP == kGram) {
+                    gram_acc<MO>(A, acc);
+                } else if constexpr (SWEEP == kDirect) {
+                    direct_acc<MO>(A, M, acc);
+                } else {
+#pragma unroll
+                    for (int t = 0; t < MO; t++) {
+                        const double2 v = cfma(w0, A[t][0], cmul(w1, A[t][1]));
+                        const S sv = Cx<S>::narrow(v);
+                        o[t][ig][ir] = sv;
+                        acc[0] += norm2(v);
+                        acc[1] += norm2(Cx<S>::widen(sv));
+                    }
                 }
-            } else {
-                const double2 v0 = cfma(w0, A[0][0], cmul(w1, A[0][1]));
-                const S s0 = Cx<S>::narrow(v0);
-                o0[i] = s0;
-                acc[0] += norm2(v0);
-                acc[1] += norm2(Cx<S>::widen(s0));
+            }
+        if constexpr (WRITES) {
+#pragma unroll
+            for (int ig = 0; ig < VG; ig++) {
+                const uint64_t a0 = ag ^ (ig ? gam : 0);
+                permute<S, VR>(o[0][ig], unsigned(a0) & (VR - 1));
+                Run<S, VR>::store(mem, a0 & ~uint64_t(VR - 1), o[0][ig]);
+                permute<S, VR>(o[0][ig], unsigned(a0) & (VR - 1));     // back to coset order for the fused norms
                 if constexpr (MO == 2) {
-                    const double2 v1 = cfma(w0, A[1][0], cmul(w1, A[1][1]));
-                    const S s1 = Cx<S>::narrow(v1);
-                    o1[i] = s1;
-                    acc[0] += norm2(v1);
-                    acc[1] += norm2(Cx<S>::widen(s1));
+                    const uint64_t a1 = a0 ^ g;
+                    permute<S, VR>(o[1][ig], unsigned(a1) & (VR - 1));
+                    Run<S, VR>::store(mem, a1 & ~uint64_t(VR - 1), o[1][ig]);
+                    permute<S, VR>(o[1][ig], unsigned(a1) & (VR - 1));
                 }
             }
         }
-        if constexpr (SWEEP == kWrite) {
-            permute<S, V>(o0, unsigned(ag) & (V - 1));
-            Run<S, V>::store(mem, ag & ~uint64_t(V - 1), o0);
-            if constexpr (MO == 2) {
-                const uint64_t a1 = ag ^ g;
-                permute<S, V>(o1, unsigned(a1) & (V - 1));
-                Run<S, V>::store(mem, a1 & ~uint64_t(V - 1), o1);
-            }
+        if constexpr (FUSED) {
+            // next step's cosets: output (ir, ig=0, tau d) pairs with output (ir, ig=1, tau d ^ eps); the element with
+            // slot index 0 is the next coset's base slot, the other its second slot.
+#pragma unroll
+            for (int ir = 0; ir < VR; ir++)
+#pragma unroll
+                for (int d = 0; d < MO; d++) {
+                    const int pl = pbase ^ plow[ir];
+                    const int jidx = ((pl >> 1) ^ (d & phig)) & 1;
+                    const int sg = ((pl >> 2) ^ (d & nug)) & 1;
+                    const double2 va = Cx<S>::widen(o[d][0][ir]);
+                    double2 vb;
+                    if constexpr (MO == 2) vb = Cx<S>::widen((d ^ eps) ? o[1][1][ir] : o[0][1][ir]);
+                    else vb = Cx<S>::widen(o[0][1][ir]);
+                    double2 w2[2];
+                    w2[0] = jidx ? vb : va;
+                    w2[1] = jidx ? va : vb;
+                    double2 A2[MO2][2];
+                    route_terms<ROUTE2, 2, MO2>(A2, w2, sg, par2mask);
+                    if constexpr (SWEEP == kWriteGram) gram_acc<MO2>(A2, acc + 2);
+                    else direct_acc<MO2>(A2, M, acc + 2);
+                }
         }
     }
-    block_sum_store<NA>(acc, part + size_t(blockIdx.x) * kPartStride);
+    sweep_epilogue<NA>(acc, part, ctl, epi, SWEEP);
 }
+
+// ---------------------------------------------------------------------------------------------------------------
+// single-thread chain kernels
+
+__global__ void decide_kernel(Ctl* ctl, Epi e, int k) {
+    Ctl& c = *ctl;
+    if (!c.dead) decide(c, e, k, -1, nullptr);
+}
+
+__global__ void publish_kernel(Ctl* ctl, Epi e) { publish(*ctl, e); }
 
 // ---------------------------------------------------------------------------------------------------------------
 // dispatch table
 
-using SweepFn = void (*)(const KStep*, void*, double*, const Ctl*);
-
-template <class S, int V, int NTQ, int ROUTE, int SWEEP>
-void* sweep_ptr() {
-    return reinterpret_cast<void*>(&sweep_kernel<S, V, NTQ, ROUTE, SWEEP>);
+template <class S, int VR, int VG, int NTQ, int ROUTE, int SWEEP, int ROUTE2>
+void* kptr() {
+    return reinterpret_cast<void*>(&sweep_kernel<S, VR, VG, NTQ, ROUTE, SWEEP, ROUTE2>);
 }
 
-template <class S, int V, int NTQ, int SWEEP>
-void* by_route(int route) {
+template <class S, int VR, int VG, int NTQ, int ROUTE, int SWEEP>
+void* by_route2(int route2) {
+    if constexpr (SWEEP == kWriteGram || SWEEP == kWriteDirect) {
+        switch (route2) {
+            case kRoute1: return kptr<S, VR, VG, NTQ, ROUTE, SWEEP, kRoute1>();
+            case kRouteSplit: return kptr<S, VR, VG, NTQ, ROUTE, SWEEP, kRouteSplit>();
+            case kRouteMixA: return kptr<S, VR, VG, NTQ, ROUTE, SWEEP, kRouteMixA>();
+            case kRouteMixB: return kptr<S, VR, VG, NTQ, ROUTE, SWEEP, kRouteMixB>();
+        }
+        return nullptr;
+    } else {
+        return kptr<S, VR, VG, NTQ, ROUTE, SWEEP, kRoute1>();
+    }
+}
+
+template <class S, int VR, int VG, int NTQ, int SWEEP>
+void* by_route(int route, int route2) {
     switch (route) {
-        case kRoute1: return sweep_ptr<S, V, NTQ, kRoute1, SWEEP>();
-        case kRouteSplit: return sweep_ptr<S, V, NTQ, kRouteSplit, SWEEP>();
-        case kRouteMixA: if constexpr (NTQ >= 2) return sweep_ptr<S, V, NTQ, kRouteMixA, SWEEP>(); else return nullptr;
-        case kRouteMixB: if constexpr (NTQ >= 2) return sweep_ptr<S, V, NTQ, kRouteMixB, SWEEP>(); else return nullptr;
+        case kRoute1: return by_route2<S, VR, VG, NTQ, kRoute1, SWEEP>(route2);
+        case kRouteSplit: return by_route2<S, VR, VG, NTQ, kRouteSplit, SWEEP>(route2);
+        case kRouteMixA: if constexpr (NTQ >= 2) return by_route2<S, VR, VG, NTQ, kRouteMixA, SWEEP>(route2); else return nullptr;
+        case kRouteMixB: if constexpr (NTQ >= 2) return by_route2<S, VR, VG, NTQ, kRouteMixB, SWEEP>(route2); else return nullptr;
     }
     return nullptr;
 }
 
-template <class S, int V, int SWEEP>
-void* by_ntq(int ntq, int route) {
+template <class S, int VR, int VG, int SWEEP>
+void* by_ntq(int ntq, int route, int route2) {
     switch (ntq) {
-        case 1: return by_route<S, V, 1, SWEEP>(route);
-        case 2: return by_route<S, V, 2, SWEEP>(route);
-        case 4: return by_route<S, V, 4, SWEEP>(route);
+        case 1: return by_route<S, VR, VG, 1, SWEEP>(route, route2);
+        case 2: return by_route<S, VR, VG, 2, SWEEP>(route, route2);
+        case 4: return by_route<S, VR, VG, 4, SWEEP>(route, route2);
     }
     return nullptr;
 }
 
 template <class S, int SWEEP>
-void* by_v(int v, int ntq, int route) {
-    if constexpr (sizeof(S) == 8) {
-        switch (v) {
-            case 0: return by_ntq<S, 1, SWEEP>(ntq, route);
-            case 2: return by_ntq<S, 4, SWEEP>(ntq, route);
-        }
+void* by_layout(int lr, int vg, int ntq, int route, int route2) {
+    constexpr bool c64 = sizeof(S) == 8;
+    if constexpr (SWEEP == kWriteGram || SWEEP == kWriteDirect) {
+        if (vg != 2) return nullptr;
+        if constexpr (c64) return lr == 1 ? by_ntq<S, 2, 2, SWEEP>(ntq, route, route2) : nullptr;
+        else return lr == 0 ? by_ntq<S, 1, 2, SWEEP>(ntq, route, route2) : nullptr;
     } else {
-        switch (v) {
-            case 0: return by_ntq<S, 1, SWEEP>(ntq, route);
-            case 1: return by_ntq<S, 2, SWEEP>(ntq, route);
-        }
+        if (vg != 1) return nullptr;
+        if (lr == 0) return by_ntq<S, 1, 1, SWEEP>(ntq, route, route2);
+        if constexpr (c64) return lr == 2 ? by_ntq<S, 4, 1, SWEEP>(ntq, route, route2) : nullptr;
+        else return lr == 1 ? by_ntq<S, 2, 1, SWEEP>(ntq, route, route2) : nullptr;
     }
-    return nullptr;
 }
 
-// Kernel for (storage: 0 = c64, 1 = c128, v, ntq, route, sweep); nullptr when that combination is not built.
-void* sweep_kernel_for(int storage, int v, int ntq, int route, int sweep) {
+// Kernel for (storage 0 = c64 / 1 = c128, log2 run, vg, ntq, route, sweep, next route); nullptr when not built.
+void* sweep_kernel_for(int storage, int lr, int vg, int ntq, int route, int sweep, int route2) {
     if (storage == 0) {
         switch (sweep) {
-            case kGram: return by_v<float2, kGram>(v, ntq, route);
-            case kDirect: return by_v<float2, kDirect>(v, ntq, route);
-            case kWrite: return by_v<float2, kWrite>(v, ntq, route);
+            case kGram: return by_layout<float2, kGram>(lr, vg, ntq, route, route2);
+            case kDirect: return by_layout<float2, kDirect>(lr, vg, ntq, route, route2);
+            case kWrite: return by_layout<float2, kWrite>(lr, vg, ntq, route, route2);
+            case kWriteGram: return by_layout<float2, kWriteGram>(lr, vg, ntq, route, route2);
+            case kWriteDirect: return by_layout<float2, kWriteDirect>(lr, vg, ntq, route, route2);
         }
     } else {
         switch (sweep) {
-            case kGram: return by_v<double2, kGram>(v, ntq, route);
-            case kDirect: return by_v<double2, kDirect>(v, ntq, route);
-            case kWrite: return by_v<double2, kWrite>(v, ntq, route);
+            case kGram: return by_layout<double2, kGram>(lr, vg, ntq, route, route2);
+            case kDirect: return by_layout<double2, kDirect>(lr, vg, ntq, route, route2);
+            case kWrite: return by_layout<double2, kWrite>(lr, vg, ntq, route, route2);
+            case kWriteGram: return by_layout<double2, kWriteGram>(lr, vg, ntq, route, route2);
+            case kWriteDirect: return by_layout<double2, kWriteDirect>(lr, vg, ntq, route, route2);
         }
     }
     return nullptr;
 }
 
-// v values built per storage (largest first): the host picks the largest the step's layout allows.
-int preferred_v(int storage) { return storage == 0 ? 2 : 1; }
+// log2 run length of the plain layout per storage, and of the fused layout (which adds the pairing coordinate).
+int plain_lr(int storage) { return storage == 0 ? 2 : 1; }
+int fused_lr(int storage) { return storage == 0 ? 1 : 0; }
+
+void* decide_kernel_ptr() { return reinterpret_cast<void*>(&decide_kernel); }
+void* publish_kernel_ptr() { return reinterpret_cast<void*>(&publish_kernel); }
 
 }  // namespace gs

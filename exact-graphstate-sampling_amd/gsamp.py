@@ -63,7 +63,8 @@ def library():
         "gs_last_error": (ctypes.c_char_p, []),
         "gs_profile_enable": (I, [P, I]),
         "gs_profile_read": (I, [P, dp, ctypes.POINTER(I64)]),
-        "gs_step_geometry": (I, [P, I, ip, ip, ip, ip, ip]),
+        "gs_step_geometry": (I, [P, I, ip]),
+        "gs_set_fusion": (I, [P, I]),
         "gs_copy_probe": (I, [I, I64, dp]),
     }
     for name, (res, args) in sig.items():
@@ -223,15 +224,21 @@ class Engine:
     def profile(self, on=True):
         _check(library().gs_profile_enable(self._h, 1 if on else 0))
 
+    SWEEPS = ("gram", "direct", "write", "write+gram", "write+direct")
+
     def profile_read(self):
-        """``(ms [ns, 3], launches [ns, 3])`` per step and sweep (gram, direct, write) since profiling was enabled."""
-        ms = np.zeros(self.ns * 3)
-        n = np.zeros(self.ns * 3, np.int64)
+        """``(ms [ns, 5], launches [ns, 5])`` per step and sweep kind (``SWEEPS``) since profiling was enabled."""
+        ms = np.zeros(self.ns * 5)
+        n = np.zeros(self.ns * 5, np.int64)
         _check(library().gs_profile_read(self._h, _f8(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
-        return ms.reshape(self.ns, 3), n.reshape(self.ns, 3)
+        return ms.reshape(self.ns, 5), n.reshape(self.ns, 5)
 
     def geometry(self, k):
-        """``(log2 cosets per thread, blocks, log2 tiles per block, ntq, route)`` of step k's sweeps."""
-        vals = [ctypes.c_int() for _ in range(5)]
-        _check(library().gs_step_geometry(self._h, k, *[ctypes.byref(v) for v in vals]))
-        return tuple(v.value for v in vals)
+        """Launch geometry of step k: dict(lr, grid, ntq, route, fusable, fgrid, route2, lg_iter)."""
+        out = np.zeros(8, np.int32)
+        _check(library().gs_step_geometry(self._h, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return dict(zip(("lr", "grid", "ntq", "route", "fusable", "fgrid", "route2", "lg_iter"), (int(x) for x in out)))
+
+    def set_fusion(self, on):
+        """Fold each sampled step's norm pass into the previous write sweep where possible (default on)."""
+        _check(library().gs_set_fusion(self._h, 1 if on else 0))

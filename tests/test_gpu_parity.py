@@ -44,11 +44,13 @@ def c1(golden):
     return d, meta, strings, hy
 
 
+@pytest.mark.parametrize("fusion", [True, False])
 @pytest.mark.parametrize("storage", [gs.C64, gs.C128])
-def test_c1_prob_chain(c1, storage):
+def test_c1_prob_chain(c1, storage, fusion):
     d, meta, strings, _ = c1
-    print(f"c1 prob/chain storage={TAG[storage]}", flush=True)
+    print(f"c1 prob/chain storage={TAG[storage]} fusion={fusion}", flush=True)
     with gs.Engine(os.path.join(d, "plan.txt"), storage) as e:
+        e.set_fusion(fusion)
         lp = e.prob(strings)
         ref = np.array([float(l[0]) for l in lines(os.path.join(d, f"prob_{TAG[storage]}.txt"))])
         assert close(lp, ref, 0) <= TOL[storage], close(lp, ref, 0)
@@ -60,12 +62,18 @@ def test_c1_prob_chain(c1, storage):
         assert close(np.log2(np.abs(amp) ** 2) - meta["ns"], lpc, 0) <= 10 * TOL[storage]
 
 
+@pytest.mark.parametrize("fusion", [True, False])
 @pytest.mark.parametrize("storage", [gs.C64, gs.C128])
 @pytest.mark.parametrize("mode", ["sample", "sample2"])
-def test_c1_sample_bits_identical(c1, storage, mode):
+def test_c1_sample_bits_identical(c1, storage, mode, fusion):
+    """Sampled bits identical to the reference; with the next step's norms folded into the previous write (default)
+    and with a separate norm sweep per sampled step."""
     d, meta, _, _ = c1
-    print(f"c1 {mode} storage={TAG[storage]}", flush=True)
+    print(f"c1 {mode} storage={TAG[storage]} fusion={fusion}", flush=True)
     with gs.Engine(os.path.join(d, "plan.txt"), storage) as e:
+        e.set_fusion(fusion)
+        if fusion:
+            assert any(e.geometry(k)["fusable"] for k in range(e.ns))
         bits, lp = e.sample(16, meta["sample_seed"], seeding=gs.SEED_PLAIN if mode == "sample" else gs.SEED_SEQ)
     ref = lines(os.path.join(d, f"{mode}_{TAG[storage]}.txt"))
     assert [egs.planio.format_bits(b) for b in bits] == [l[0] for l in ref]
