@@ -30,7 +30,7 @@ def _run(cmd, quiet_warn=True):
 def build_library():
     os.makedirs(BIN, exist_ok=True)
     objs = []
-    for src in ("gs_kernels.hip", "gs_engine.hip", "gs_plan.cpp"):
+    for src in ("gs_kernels.hip", "gs_engine.hip", "gs_batch.hip", "gs_plan.cpp"):
         obj = os.path.join(BIN, src.split(".")[0] + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
         _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])

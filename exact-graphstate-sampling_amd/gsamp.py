@@ -19,7 +19,9 @@ SEED_PLAIN, SEED_SEQ = 0, 1
 # every symbol include/gsamp_amd.h declares
 ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy", "gs_engine_create",
                "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
-               "gs_last_device_ms", "gs_last_error")
+               "gs_last_device_ms", "gs_last_error", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
+               "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_last_ms", "gs_batch_last_error",
+               "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms")
 
 _lib = None
 
@@ -66,6 +68,17 @@ def library():
         "gs_step_geometry": (I, [P, I, ip]),
         "gs_set_fusion": (I, [P, I]),
         "gs_copy_probe": (I, [I, I64, dp]),
+        "gs_batch_create": (I, [ctypes.POINTER(P), ip, I, I, I, pp]),
+        "gs_batch_set_matrices": (I, [P, I, dp]),
+        "gs_batch_grad": (I, [P, ctypes.POINTER(i8p), ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp), I, D,
+                              ip]),
+        "gs_batch_grad_ms": (D, [P, dp]),
+        "gs_batch_destroy": (None, [P]),
+        "gs_batch_score": (I, [P, ctypes.POINTER(i8p), ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp)]),
+        "gs_batch_use_graph": (I, [P, I]),
+        "gs_batch_set_tile": (I, [P, I, I, I]),
+        "gs_batch_last_ms": (D, [P, dp]),
+        "gs_batch_last_error": (ctypes.c_char_p, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -242,3 +255,103 @@ class Engine:
     def set_fusion(self, on):
         """Fold each sampled step's norm pass into the previous write sweep where possible (default on)."""
         _check(library().gs_set_fusion(self._h, 1 if on else 0))
+
+
+class BatchEngine:
+    """Batched chain-mode scoring of many strings of many plans, one launch per step (``gs_batch_*``).
+
+    ``plans``: list of Plan objects or paths; ``lanes``: strings per plan per call (multiples of 64)."""
+
+    def __init__(self, plans, lanes, storage=C64, device=0):
+        lib = library()
+        self.plans = [p if isinstance(p, Plan) else Plan(path=p) for p in plans]
+        self.lanes = [int(x) for x in lanes]
+        self.storage = storage
+        arr = (ctypes.c_void_p * len(self.plans))(*[p._h for p in self.plans])
+        ln = (ctypes.c_int * len(self.lanes))(*self.lanes)
+        self._h = ctypes.c_void_p()
+        rc = lib.gs_batch_create(arr, ln, len(self.plans), storage, device, ctypes.byref(self._h))
+        if rc:
+            raise GsError(rc, lib.gs_batch_last_error().decode())
+        self.maxns = max(p.ns for p in self.plans)
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.gs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc:
+            raise GsError(rc, library().gs_batch_last_error().decode())
+
+    def use_graph(self, on=True):
+        self._check(library().gs_batch_use_graph(self._h, 1 if on else 0))
+
+    def set_tile(self, i, lg, k=-1):
+        self._check(library().gs_batch_set_tile(self._h, i, k, lg))
+
+    def score(self, bits, want_cond=True, want_amp=True):
+        """bits: list (per plan) of int8 [lanes_i, ns_i].  Returns lists of log2p [lanes_i], cond [lanes_i, ns_i],
+        amp [lanes_i] complex."""
+        n = len(self.plans)
+        bs = [np.ascontiguousarray(np.asarray(b, dtype=np.int8)) for b in bits]
+        for b, p, ln in zip(bs, self.plans, self.lanes):
+            if b.shape != (ln, p.ns):
+                raise ValueError(f"bits must be [{ln}, {p.ns}], got {b.shape}")
+        lp = [np.empty(ln) for ln in self.lanes]
+        cd = [np.empty((ln, p.ns)) for ln, p in zip(self.lanes, self.plans)]
+        am = [np.empty((ln, 2)) for ln in self.lanes]
+        I8P = ctypes.POINTER(ctypes.c_int8)
+        DP = ctypes.POINTER(ctypes.c_double)
+        bp = (I8P * n)(*[_i8(b) for b in bs])
+        lpp = (DP * n)(*[_f8(x) for x in lp])
+        cdp = (DP * n)(*[_f8(x) for x in cd]) if want_cond else None
+        amp = (DP * n)(*[_f8(x) for x in am]) if want_amp else None
+        self._check(library().gs_batch_score(self._h, bp, lpp, cdp, amp))
+        return lp, cd, [a[:, 0] + 1j * a[:, 1] for a in am]
+
+    def last_ms(self):
+        """``(device ms of the last score, end of each step in ms after its start or -1)``."""
+        st = np.zeros(self.maxns)
+        ms = library().gs_batch_last_ms(self._h, _f8(st))
+        return ms, st
+
+    def set_matrices(self, i, M):
+        """Instance i's measurement matrices: complex array [ns, 2, 2] (M[k, b, x])."""
+        M = np.asarray(M, dtype=complex)
+        if M.shape != (self.plans[i].ns, 2, 2):
+            raise ValueError("M must be [ns, 2, 2]")
+        flat = np.ascontiguousarray(np.stack([M.real, M.imag], axis=-1).reshape(-1))
+        self._check(library().gs_batch_set_matrices(self._h, i, _f8(flat)))
+
+    def grad(self, bits, seg=0, mem_budget=64e9):
+        """Adjoint: per plan ``(log2p [B], Z [B] complex, G [B, ns, 2] complex)`` with G[b, k, x] = dZ_b/dM_k[b_k, x];
+        checkpoint segments of ``seg`` steps (0: largest fitting ``mem_budget`` bytes).  Returns (results, seg)."""
+        n = len(self.plans)
+        bs = [np.ascontiguousarray(np.asarray(b, dtype=np.int8)) for b in bits]
+        lp = [np.empty(ln) for ln in self.lanes]
+        am = [np.empty((ln, 2)) for ln in self.lanes]
+        G = [np.empty((ln, p.ns, 2, 2)) for ln, p in zip(self.lanes, self.plans)]
+        I8P = ctypes.POINTER(ctypes.c_int8)
+        DP = ctypes.POINTER(ctypes.c_double)
+        used = ctypes.c_int()
+        self._check(library().gs_batch_grad(self._h, (I8P * n)(*[_i8(b) for b in bs]), (DP * n)(*[_f8(x) for x in lp]),
+                                            (DP * n)(*[_f8(x) for x in am]), (DP * n)(*[_f8(x) for x in G]), seg,
+                                            mem_budget, ctypes.byref(used)))
+        res = [(lp[i], am[i][:, 0] + 1j * am[i][:, 1], G[i][..., 0] + 1j * G[i][..., 1]) for i in range(n)]
+        return res, used.value
+
+    def grad_ms(self):
+        """``(total, forward)`` device ms of the last grad call."""
+        f = ctypes.c_double()
+        t = library().gs_batch_grad_ms(self._h, ctypes.byref(f))
+        return t, f.value

@@ -11,7 +11,11 @@ An instance is a circuit on n qubits:
 The last body layer of ``h`` makes every tail start with a non-Clifford gate, so ``clifford_body`` of the written
 QASM splits body and tail exactly where this module does.  Optional phase-gadget ancillas (a T gate on the parity
 of three random vertices, realised as an ancilla a with ``h a; cz q,a (x3); h a`` in the body and ``t a; h a`` in
-its tail) model doped circuits: projecting every ancilla onto 0 applies the gadgets.
+its tail) model doped circuits: projecting every ancilla onto 0 applies the gadgets.  Gadgets attach to vertices
+among the first ``band`` data vertices and the ancillas come first in the order, so every projected vertex precedes
+the sampled data steps (the condition under which hybrid sampling of the data suffix is exact, gsamp2.cpp:71-74) and
+the width stays at most ``band``: after the ancillas, every cut's future side touched by processed vertices lies in a
+window of ``band`` data vertices.
 """
 import numpy as np
 
@@ -68,7 +72,7 @@ def banded_ops(n, band, seed, gadget_count=0, gadget_arity=3):
     beta = rng.uniform(0, 2 * np.pi, size=n)
     anc = []
     for a in range(gadget_count):
-        qs = sorted(int(q) for q in rng.choice(n, size=gadget_arity, replace=False))
+        qs = sorted(int(q) for q in rng.choice(min(n, band), size=gadget_arity, replace=False))
         av = n + a
         body += [("h", [av])] + [("cz", [q, av]) for q in qs] + [("h", [av])]
         anc.append((av, qs))

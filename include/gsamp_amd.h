@@ -81,6 +81,33 @@ int gs_resume(gs_engine* eng, const int8_t* bits, int K, int nslices, const doub
               const double* logscales, const uint64_t* cshift, const uint64_t* tqshift, uint64_t seed, int64_t line,
               int8_t* out_bits, double* log2p_full, double* log2p_prefix, double* cond);
 
+/* ---- batched scoring (no counterpart in the reference: chain mode, gsamp2.cpp:56-70, for many strings and many
+ * plans at once).  Instance i holds lanes[i] strings (a multiple of 64) in an arena of 2^R_i x lanes[i] complex64
+ * entries, slot-major and batch-minor; one launch per step index covers every instance. */
+typedef struct gs_batch gs_batch;
+int gs_batch_create(const gs_plan* const* plans, const int* lanes, int nplans, int storage, int device, gs_batch** out);
+void gs_batch_destroy(gs_batch* batch);
+/* bits[i]: instance i's [lanes_i][ns_i] forced strings; log2p[i] [lanes_i], cond[i] [lanes_i][ns_i] and
+ * amp[i] [lanes_i][2] (cond, amp or their entries may be NULL) receive chain-mode outputs. */
+int gs_batch_score(gs_batch* batch, const int8_t* const* bits, double* const* log2p, double* const* cond,
+                   double* const* amp);
+/* Replay the step sequence as one captured hipGraph (on) or launch it kernel by kernel (off, default). */
+int gs_batch_use_graph(gs_batch* batch, int on);
+/* log2 cosets per work item of instance i at step k (k < 0: every step); not below the creation default. */
+int gs_batch_set_tile(gs_batch* batch, int i, int k, int lg);
+/* Device time of the last gs_batch_score (ms); step_end_ms (may be NULL) [max ns]: end of each step after the start. */
+double gs_batch_last_ms(const gs_batch* batch, double* step_end_ms);
+/* Replace instance i's measurement matrices M[ns][2][2] (complex as re, im pairs; M[k][b][x]). */
+int gs_batch_set_matrices(gs_batch* batch, int i, const double* M);
+/* Adjoint (no counterpart in the reference): G[i] [lanes_i][ns_i][2] complex, G_k(x) = dZ/dM_k[b_k, x] for every
+ * string, with checkpoint-and-recompute of the in-place table every seg steps (seg 0: the largest segment whose
+ * copies fit in mem_budget bytes); also log2p[i] and amp[i] (may be NULL).  *seg_used receives the segment. */
+int gs_batch_grad(gs_batch* batch, const int8_t* const* bits, double* const* log2p, double* const* amp,
+                  double* const* G, int seg, double mem_budget, int* seg_used);
+/* Device time (ms) of the last gs_batch_grad; *forward_ms (may be NULL) its first forward pass. */
+double gs_batch_grad_ms(const gs_batch* batch, double* forward_ms);
+const char* gs_batch_last_error(void);
+
 /* Timing of the device work of the last call, in milliseconds (HIP events around the step kernels). */
 double gs_last_device_ms(const gs_engine* eng);
 

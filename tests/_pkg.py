@@ -15,7 +15,8 @@ def _load():
     mod = importlib.util.module_from_spec(spec)
     sys.modules["egs"] = mod
     spec.loader.exec_module(mod)
-    for name in ("tableau", "graphform", "amp", "plan", "inplace", "engine", "gadgets", "instances", "planio", "gsamp"):
+    for name in ("tableau", "graphform", "amp", "plan", "inplace", "engine", "gadgets", "instances", "planio", "gsamp",
+                 "adjoint"):
         importlib.import_module(f"egs.{name}")
     return mod
 
