@@ -10,7 +10,10 @@ The JSON line also carries:
   cpu_baseline  -- one sample of the same plan by the reference executor compiled from its sources (oracle/_ref), or
                    this project's CPU restatement (oracle/gs_oracle) when that is absent, on the host's cores;
   c64_vs_c128   -- the same samples' seeds in a complex128 arena: speedup and max |d log2 Pr| (c64 bits re-scored
-                   in c128 with prob mode).
+                   in c128 with prob mode);
+  ibm_w30       -- exact samples of the reference's own width-30 plan of the 70-qubit IBM circuit
+                   (tests/golden/ibm/nq70_inplace_plan.txt, Sum_k 2^r_k = 2^33), beside the reference's published
+                   11.3 s per sample for that plan (README.md:24, laptop CPU).
 """
 import argparse
 import json
@@ -43,12 +46,25 @@ def sweep_bytes(step, sweep, esz):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "c5"),
+                    help="c2: exact sampling at width 30 (the headline line); c3 batched scoring; c4 gradients; "
+                         "c5 width-34 sampling (each prints its own JSON line)")
+    ap.add_argument("--steps", type=int, default=256)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--c128-steps", type=int, default=8)
+    ap.add_argument("--cpu-samples", type=int, default=8, help="samples in the bounded CPU baseline run")
+    ap.add_argument("--c128-steps", type=int, default=16)
+    ap.add_argument("--ibm-steps", type=int, default=8, help="samples of the IBM width-30 plan (0: skip)")
+    ap.add_argument("--c3-strings", type=int, default=8192)
+    ap.add_argument("--c3-lanes", type=int, default=512)
+    ap.add_argument("--c3-single", type=int, default=256, help="strings timed one per launch sequence")
+    ap.add_argument("--c3-budget", type=float, default=120.0, help="GB of arena per batch in the mixed schedule")
+    ap.add_argument("--c5-samples", type=int, default=32)
+    ap.add_argument("--c5-strings", type=int, default=256)
+    ap.add_argument("--c5-c128", type=int, default=4, help="c5 samples/strings re-scored in a complex128 arena")
     ap.add_argument("--no-fusion", action="store_true", help="run every norm pass as its own sweep")
+    ap.add_argument("--grid-log2", type=int, default=0, help="cap sweeps at 2^N blocks (default: engine default)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="first run one plain device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -59,6 +75,10 @@ def main():
 
     from _pkg import egs
     gs = egs.gsamp
+    if args.workload != "c2":
+        out = {"c3": workload_c3, "c4": workload_c4, "c5": workload_c5}[args.workload](gs, egs, args)
+        print(json.dumps(out))
+        return
     if args.calibrate_copy:
         ms = gs.copy_probe(args.calibrate_copy << 20)
         print(f"copy probe: {args.calibrate_copy} MiB in {ms:.3f} ms = {2 * (args.calibrate_copy << 20) / ms / 1e6:.1f} GB/s "
@@ -76,6 +96,8 @@ def main():
     with gs.Engine(plan_path, gs.C64) as e:
         if args.no_fusion:
             e.set_fusion(False)
+        if args.grid_log2:
+            e.set_grid_log2(args.grid_log2)
         if W:
             e.sample(W, args.seed + 10 ** 6)
         e.profile(True)
@@ -106,6 +128,7 @@ def main():
             a["n"] += int(nl[k, sw])
             a["bytes"] += sweep_bytes(s, sw, 8) * nl[k, sw]
             a["steps"].append(k)
+    moved = sum(a["bytes"] for a in agg.values()) / K           # bytes the sweeps actually stream per sample
     key, dom = max(agg.items(), key=lambda kv: kv[1]["ms"])
     achieved = dom["bytes"] / (dom["ms"] / 1e3) / 1e9
     vr, vg, ntq, route, sw, route2 = key
@@ -128,9 +151,11 @@ def main():
     dlog = float(np.max(np.abs(rescored - lp[:K2])))
     same_bits = int(sum(int(np.array_equal(a, b)) for a, b in zip(bits[:K2], bits2)))
 
+    ibm = ibm_w30(gs, egs, args.ibm_steps, args) if args.ibm_steps else None
+
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(plan_path, args.seed)
+        cpu = cpu_baseline(plan_path, args.seed, args.cpu_samples)
 
     out = {
         "metric": "s/exact sample (n=70 band-30 banded graph state, 2^30 in-place arena, complex64 + FP64 sums)",
@@ -150,6 +175,9 @@ def main():
                                f"{inst.width}, R=30, 1 sample per step",
                    "n": 70, "band": 30, "R": R, "width": inst.width, "log2_sum_2r": round(inst.log2_sum_live(), 3)},
         "hbm_gbps_algorithmic": traffic["sample"] / s_per_sample / 1e9,
+        "gb_per_sample_algorithmic": traffic["sample"] / 1e9,
+        "gb_per_sample_moved": moved / 1e9,
+        "hbm_gbps_moved": moved / s_per_sample / 1e9,
         "device_ms_per_sample": dev_ms / K,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_meas, "kernel": kname, "kernel_desc": kdesc,
@@ -159,8 +187,196 @@ def main():
         "c64_vs_c128": {"c128_s_per_sample": wall2, "speedup": wall2 / s_per_sample,
                         "max_abs_dlog2p": dlog, "samples_compared": K2, "identical_bits": same_bits},
         "cpu_baseline": cpu,
+        "ibm_w30": ibm,
     }
     print(json.dumps(out))
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload_c3(gs, egs, args):
+    """Batched scoring.  (a) 8192 strings on the n = 70, band-26 instance (seed 3): half uniform, half exact samples
+    drawn by the sequential sampler; batched chain-mode scoring (arena of 2^w x B lanes, tiles from the autotuner)
+    against one string per launch sequence (the single-string engine's chain mode).  (b) 96 instances, n in
+    {40, 56, 70} with bands {16, 22, 26} (32 each, seeds 300..395), 64 strings each, scheduled into batches under a
+    memory budget by the autotuner; p50 / p99 per-instance latency."""
+    at = egs.autotune
+    tmp = tempfile.mkdtemp(prefix="gs_c3_")
+    cache = os.path.join(tmp, "tiles.json")
+    inst = egs.instances.config_instance("c3")
+    path = os.path.join(tmp, "c3.txt")
+    inst.write_plan(path)
+    ns = len(inst.steps)
+    N, B = args.c3_strings, args.c3_lanes
+    rng = np.random.default_rng(3)
+    strings = rng.integers(0, 2, size=(N, ns)).astype(np.int8)
+    res = {"workload": "c3 batched scoring", "width": inst.width, "strings": N, "lanes": B}
+    with gs.Engine(path, gs.C64) as e:
+        t0 = time.perf_counter()
+        strings[N // 2:], _ = e.sample(N - N // 2, 33)
+        log(f"c3: {N - N // 2} exact samples drawn in {time.perf_counter() - t0:.1f} s")
+        n1 = min(N, args.c3_single)
+        e.chain(strings[:4])
+        t0 = time.perf_counter()
+        lp1, _, _ = e.chain(strings[:n1])
+        t1 = (time.perf_counter() - t0) / n1
+    log(f"c3: one string per launch sequence: {1 / t1:.0f} amplitudes/s")
+    with gs.BatchEngine([path], [B], storage=gs.C64) as bt:
+        tt = at.tune_tiles(bt, [strings[:B]], cache=cache)
+        bt.use_graph(True)
+        bt.score([strings[:B]], want_cond=False, want_amp=False)
+        lps = []
+        t0 = time.perf_counter()
+        for i in range(0, N, B):
+            lp, _, _ = bt.score([strings[i:i + B]], want_cond=False, want_amp=False)
+            lps.append(lp[0])
+        tb = time.perf_counter() - t0
+    lpb = np.concatenate(lps)
+    res["single"] = {"amplitudes_per_s": 1 / t1, "strings_timed": n1}
+    res["batched"] = {"amplitudes_per_s": N / tb, "tiles": tt, "speedup_vs_single": (N / tb) * t1,
+                      "max_abs_dlog2p_vs_single": float(np.max(np.abs(lpb[:n1] - lp1)))}
+    log(f"c3 batched: {N / tb:.0f} amplitudes/s")
+    # mixed instances
+    plans, widths, bits = [], [], []
+    for j, (n, band) in enumerate([(40, 16)] * 32 + [(56, 22)] * 32 + [(70, 26)] * 32):
+        mi = egs.instances.Instance(n, band, 300 + j)
+        pth = os.path.join(tmp, f"m{j}.txt")
+        mi.write_plan(pth)
+        plans.append(gs.Plan(path=pth))
+        widths.append(mi.R)
+        bits.append(np.random.default_rng(1000 + j).integers(0, 2, size=(64, len(mi.steps))).astype(np.int8))
+    best, sched, _ = at.tune_schedule(gs, plans, widths, 64, bits, args.c3_budget * 1e9, gs.C64, tile_cache=cache)
+    res["mixed"] = {"instances": len(plans), "lanes": 64, "widths": [min(widths), max(widths)],
+                    "budget_gb": args.c3_budget, "best_schedule": best, "schedules": sched}
+    log(f"c3 mixed: {best} {sched[best]}")
+    return res
+
+
+def workload_c4(gs, egs, args):
+    """Gradients: n = 60, band 24 (seed 4), 120 trainable angles (alpha, beta per qubit); loss = mean -ln Pr(b) over
+    1024 seeded uniform strings; forward = batched chain scoring, backward = the adjoint with checkpoint-and-recompute."""
+    inst = egs.instances.config_instance("c4")
+    model = egs.adjoint.AngleModel(inst)
+    path = os.path.join(tempfile.mkdtemp(prefix="gs_c4_"), "c4.txt")
+    inst.write_plan(path)
+    B = 1024
+    bits = np.random.default_rng(4).integers(0, 2, size=(B, len(inst.steps))).astype(np.int8)
+    log(f"c4: n=60 band 24 width {inst.width}, {B} strings")
+    res = {"workload": "c4 gradients, n=60 band 24 (seed 4), 120 angles, loss over 1024 strings", "width": inst.width}
+    for storage, tag in ((gs.C64, "c64"), (gs.C128, "c128")):
+        with gs.BatchEngine([path], [B], storage=storage) as bt:
+            bt.use_graph(True)
+            M, dA, dB = model.matrices(model.alpha0, model.beta0)
+            bt.set_matrices(0, M)
+            bt.score([bits], want_cond=False, want_amp=False)
+            t0 = time.perf_counter()
+            reps = 5
+            for _ in range(reps):
+                lp, _, _ = bt.score([bits], want_cond=False, want_amp=False)
+            fwd_wall = (time.perf_counter() - t0) / reps
+            fwd_dev = bt.last_ms()[0]
+            bt.grad([bits])
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                (r,), seg = bt.grad([bits])
+            grad_wall = (time.perf_counter() - t0) / reps
+            tot_dev, gfwd_dev = bt.grad_ms()
+            ga, gb = model.grad_ln_pr(bits, r[1], r[2], dA, dB)
+            loss = float(-np.mean(lp[0]) * np.log(2))
+        res[tag] = {"loss": loss, "forward_ms": 1e3 * fwd_wall, "forward_device_ms": fwd_dev,
+                    "grad_ms": 1e3 * grad_wall, "grad_device_ms": tot_dev, "grad_forward_part_device_ms": gfwd_dev,
+                    "backward_over_forward": (tot_dev - gfwd_dev) / fwd_dev, "grad_over_forward": tot_dev / fwd_dev,
+                    "grads_per_s": 1.0 / grad_wall, "string_grads_per_s": B / grad_wall, "checkpoint_segment": seg,
+                    "grad_norm": float(np.sqrt(np.sum((-gb.mean(0)) ** 2) + np.sum((-ga.mean(0)) ** 2)))}
+        log(f"c4 {tag}: {res[tag]}")
+    return res
+
+
+def workload_c5(gs, egs, args):
+    """Width 34: n = 80, band 34 (seed 5) plus 40 phase-gadget ancillas projected on 0, complex64 arena of 2^34
+    entries (137 GB).  Exact samples of the data qubits (hybrid mode: ancillas forced, data sampled) and chain-mode
+    log2 Pr of random strings; a complex128 rerun (2^34 x 16 B = 275 GB) on a subset gives |d log2 Pr|."""
+    t0 = time.perf_counter()
+    inst = egs.instances.config_instance("c5")
+    path = os.path.join(tempfile.mkdtemp(prefix="gs_c5_"), "c5.txt")
+    inst.write_plan(path)
+    _, R, steps = egs.planio.read_inplace_plan(path)
+    traffic = egs.planio.plan_traffic(steps, 8)
+    nanc = inst.N - inst.n
+    ns = len(steps)
+    log(f"c5: N={inst.N} width {inst.width} R={R} ({time.perf_counter() - t0:.0f} s to plan), "
+        f"{traffic['sample'] / 1e12:.2f} TB algorithmic per sample")
+    nsamp, nstr, nsub = args.c5_samples, args.c5_strings, args.c5_c128
+    hy = np.full((nsamp, ns), -1, np.int8)
+    hy[:, :nanc] = 0                                            # ancillas are the first nanc steps
+    rnd = np.random.default_rng(55).integers(0, 2, size=(nstr, ns)).astype(np.int8)
+    rnd[:, :nanc] = 0
+    res = {"workload": "c5 width 34: n=80 band 34 seed 5 + 40 gadget ancillas", "N": inst.N, "width": inst.width,
+           "R": R, "tb_per_sample_algorithmic": traffic["sample"] / 1e12}
+    with gs.Engine(path, gs.C64) as e:
+        obs, lpss, lpcs = [], [], []
+        ts = tc = 0.0
+        for i in range(0, nsamp, 8):                            # chunks, so progress shows every few seconds
+            t0 = time.perf_counter()
+            ob, lp, _ = e.hybrid(hy[i:i + 8], 5, i)
+            ts += time.perf_counter() - t0
+            obs.append(ob)
+            lpss.append(lp)
+            log(f"c5 c64: samples {i}..{i + len(lp) - 1} done, {ts / (i + len(lp)):.3f} s/sample")
+        for i in range(0, nstr, 16):
+            t0 = time.perf_counter()
+            lp, _, _ = e.chain(rnd[i:i + 16])
+            tc += time.perf_counter() - t0
+            lpcs.append(lp)
+            log(f"c5 c64: strings {i}..{i + len(lp) - 1} done, {tc / (i + len(lp)):.3f} s/string")
+        ob, lps, lpc = np.concatenate(obs), np.concatenate(lpss), np.concatenate(lpcs)
+        ts /= nsamp
+        tc /= nstr
+    res["c64"] = {"s_per_sample": ts, "samples": nsamp, "hbm_gbps_algorithmic": traffic["sample"] / ts / 1e9,
+                  "s_per_string_chain": tc, "strings": nstr,
+                  "chain_gbps_algorithmic": traffic["chain1"] / tc / 1e9}
+    if nsub:
+        try:
+            with gs.Engine(path, gs.C128) as e2:
+                t0 = time.perf_counter()
+                lp2 = e2.prob(ob[:nsub])
+                t2 = (time.perf_counter() - t0) / nsub
+                lpc2, _, _ = e2.chain(rnd[:nsub])
+            res["c128"] = {"strings": nsub, "s_per_string_prob": t2,
+                           "max_abs_dlog2p_samples": float(np.max(np.abs(lp2 - lps[:nsub]))),
+                           "max_abs_dlog2p_strings": float(np.max(np.abs(lpc2 - lpc[:nsub])))}
+        except gs.GsError as ex:
+            res["c128"] = {"error": str(ex)}
+    return res
+
+
+IBM_PLAN = os.path.join("tests", "golden", "ibm", "nq70_inplace_plan.txt")
+IBM_PUBLISHED_S = 11.3      # README.md:24: 1,012 samples in 190 min, gsamp2 sample2, Ryzen 9 9955HX laptop
+
+
+def ibm_w30(gs, egs, K, args):
+    """K exact samples (seeds 2000000..) of the reference's width-30 in-place plan of the IBM #151 circuit."""
+    path = os.path.join(ROOT, IBM_PLAN)
+    _, R, steps = egs.planio.read_inplace_plan(path)
+    traffic = egs.planio.plan_traffic(steps, 8)
+    with gs.Engine(path, gs.C64) as e:
+        if args.no_fusion:
+            e.set_fusion(False)
+        if args.grid_log2:
+            e.set_grid_log2(args.grid_log2)
+        e.sample(1, 7)
+        t0 = time.perf_counter()
+        bits, lp = e.sample(K, 2000000)
+        wall = (time.perf_counter() - t0) / K
+    return {"plan": IBM_PLAN, "R": R, "samples": K, "s_per_sample": wall,
+            "gb_per_sample_algorithmic": traffic["sample"] / 1e9,
+            "hbm_gbps_algorithmic": traffic["sample"] / wall / 1e9,
+            "reference_published_s_per_sample": IBM_PUBLISHED_S,
+            "reference_published_hardware": "laptop, AMD Ryzen 9 9955HX (README.md:24)",
+            "speedup_vs_published": IBM_PUBLISHED_S / wall,
+            "first_sample": [egs.planio.format_bits(bits[0]), float(lp[0])]}
 
 
 TRAFFIC_PROFILE = "profiles/sweep_c2_hbm_traffic.md"
@@ -179,8 +395,9 @@ def profiled_traffic(kname):
     return None
 
 
-def cpu_baseline(plan_path, seed):
-    """One exact sample of the same plan on the host cores; reference executor if it was built, else the port."""
+def cpu_baseline(plan_path, seed, count):
+    """``count`` exact samples of the same plan on the host cores (one process, plan parsed once); the reference
+    executor if it was built, else the port."""
     ref = os.path.join(ROOT, "oracle", "_ref", "gsamp2")
     port = os.path.join(ROOT, "oracle", "gs_oracle")
     exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
@@ -189,7 +406,7 @@ def cpu_baseline(plan_path, seed):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     env = dict(os.environ, OMP_NUM_THREADS=str(cores))
     t0 = time.perf_counter()
-    res = subprocess.run([exe, plan_path, "sample", "1", str(seed)], capture_output=True, text=True, env=env,
+    res = subprocess.run([exe, plan_path, "sample", str(count), str(seed)], capture_output=True, text=True, env=env,
                          timeout=600)
     wall = time.perf_counter() - t0
     if res.returncode:
@@ -200,8 +417,9 @@ def cpu_baseline(plan_path, seed):
             cpu_model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
     except (OSError, StopIteration):
         pass
-    return {"value": wall, "unit": "s/sample", "cores": cores, "kind": kind,
-            "sample": "1 exact sample (seed %d) of the same plan, whole process wall time incl. plan parse" % seed,
+    return {"value": wall / count, "unit": "s/sample", "cores": cores, "kind": kind,
+            "sample": "%d exact samples (seeds %d..%d) of the same plan, whole process wall time incl. plan parse, "
+                      "divided by %d" % (count, seed, seed + count - 1, count),
             "cpu": cpu_model, "line": res.stdout.strip()[:200]}
 
 

@@ -8,8 +8,12 @@
 namespace gs {
 
 constexpr int kThreads = 256;       // threads per sweep block (4 waves of 64)
-constexpr int kMaxGridLog2 = 11;    // at most 2048 blocks per sweep; each loops over tiles
+constexpr int kMaxGridLog2 = 14;    // at most 16384 blocks per sweep; beyond that each block loops over tiles
+constexpr int kDefaultGridLog2 = 11;
+constexpr int kFenceBlocks = 64;    // sweeps of at most this many blocks reduce in their last block
 constexpr int kPartStride = 8;      // doubles per block partial (write norms + fused next-step norms)
+constexpr size_t kPartDoubles = size_t(kPartStride) << kMaxGridLog2;
+constexpr int kCounters = 1;
 
 // Canonical routing of the 2 * ntq terms of a step onto the outputs tau (after the host relabels the offsets):
 //   kRoute1     m = 1: every term feeds tau 0
@@ -78,7 +82,7 @@ struct Epi {
     int8_t* bits;              // this string's output bits
     double* cond;              // this string's conditionals
     double* out;               // this string's scalars: logp, logscale, prev, dead
-    unsigned* counter;         // blocks finished (the last block resets it)
+    unsigned* counter;         // blocks finished (in-kernel reduction; the last block resets it)
     const KStep* plain;        // plain descriptors (M, growth flags) indexed by step
 };
 

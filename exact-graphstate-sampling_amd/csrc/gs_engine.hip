@@ -32,6 +32,7 @@ int plain_lr(int storage);
 int fused_lr(int storage);
 void* decide_kernel_ptr();
 void* publish_kernel_ptr();
+void* reduce_kernel_ptr(int sweep);
 }  // namespace gs
 
 using namespace gs;
@@ -241,6 +242,7 @@ struct gs_engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
     bool allow_fusion = true;
+    int grid_cap = kDefaultGridLog2;  // log2 of the most blocks a sweep launches
     // optional per-sweep timing (bench / profiling)
     bool profile = false;
     std::vector<hipEvent_t> pev;
@@ -268,19 +270,26 @@ int launch_sweep(gs_engine* e, int k, int sweep, const Epi& epi) {
     Ctl* ctl = e->ctl;
     Epi ep = epi;
     void* args[] = {&sp, &mem, &part, &ctl, &ep};
+    hipEvent_t a = nullptr, b = nullptr;
     if (e->profile) {
-        hipEvent_t a, b;
         HIP_TRY(hipEventCreate(&a));
         HIP_TRY(hipEventCreate(&b));
         HIP_TRY(hipEventRecord(a, e->stream));
-        HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kThreads), args, 0, e->stream));
+    }
+    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kThreads), args, 0, e->stream));
+    if (e->profile) {                   // the sweep kernel alone (the reduce kernel, if any, is not timed here)
         HIP_TRY(hipEventRecord(b, e->stream));
         e->pev.push_back(a);
         e->pev.push_back(b);
         e->pev_tag.push_back(k * 8 + sweep);
-        return 0;
     }
-    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kThreads), args, 0, e->stream));
+    if (grid > kFenceBlocks) {          // the blocks only stored their partials: reduce and run the chain
+        const double* cpart = part;
+        unsigned nblk = unsigned(grid);
+        int sw = sweep;
+        void* rargs[] = {&cpart, &nblk, &ctl, &ep, &sw};
+        HIP_TRY(hipLaunchKernel(reduce_kernel_ptr(sweep), dim3(1), dim3(kThreads), rargs, 0, e->stream));
+    }
     return 0;
 }
 
@@ -418,14 +427,14 @@ bool gf2_solve(const std::vector<uint64_t>& basis, uint64_t x, uint64_t& coef) {
 
 int parity64(uint64_t x) { return __builtin_parityll(x); }
 
-void set_geometry(KStep& ks, int lv) {
+void set_geometry(KStep& ks, int lv, int cap) {
     const int lt = lv + 8;
     if (ks.nk <= lt) {
         ks.lg_grid = 0;
         ks.lg_iter = 0;
     } else {
         const int tiles = ks.nk - lt;
-        ks.lg_grid = std::min(tiles, kMaxGridLog2);
+        ks.lg_grid = std::min(tiles, cap);
         ks.lg_iter = tiles - ks.lg_grid;
     }
 }
@@ -502,7 +511,7 @@ bool build_fused(const gs_engine* e, int k, KStep& f) {
     f.eps = eps;
     f.phig = phig;
     f.nug = nug;
-    set_geometry(f, lrf + 1);
+    set_geometry(f, lrf + 1, e->grid_cap);
     return true;
 }
 
@@ -536,7 +545,7 @@ int build_steps(gs_engine* e) {
         int unit = 0;
         while (unit < si.nk && unit < lrmax && si.cols[size_t(unit)] == (uint64_t(1) << unit)) unit++;
         ks.lr = unit == lrmax ? lrmax : 0;
-        set_geometry(ks, ks.lr);
+        set_geometry(ks, ks.lr, e->grid_cap);
         auto& L = e->launch[size_t(k)];
         L.grid = 1 << ks.lg_grid;
         L.gram = sweep_kernel_for(e->storage, ks.lr, 1, ks.ntq, ks.route, kGram, 0);
@@ -736,8 +745,8 @@ int gs_engine_create(const gs_plan* plan, int storage, int device, gs_engine** o
     const size_t C = gs_engine::kChunk;
     const size_t nsz = size_t(std::max(ns, 1));
     bool ok = dev_alloc((void**)&e->dplain, sizeof(KStep) * nsz) && dev_alloc((void**)&e->dfused, sizeof(KStep) * nsz) &&
-              dev_alloc((void**)&e->part, sizeof(double) * kPartStride << kMaxGridLog2) &&
-              dev_alloc((void**)&e->counter, sizeof(unsigned)) &&
+              dev_alloc((void**)&e->part, sizeof(double) * kPartDoubles) &&
+              dev_alloc((void**)&e->counter, sizeof(unsigned) * kCounters) &&
               dev_alloc((void**)&e->ctl, sizeof(Ctl)) && dev_alloc((void**)&e->d_forced, C * nsz) &&
               dev_alloc((void**)&e->d_U, sizeof(double) * C * nsz) && dev_alloc((void**)&e->d_bits, C * nsz) &&
               dev_alloc((void**)&e->d_cond, sizeof(double) * C * nsz) &&
@@ -752,7 +761,7 @@ int gs_engine_create(const gs_plan* plan, int storage, int device, gs_engine** o
     if (ns > 0 && (hipMemcpy(e->dplain, e->hplain.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess ||
                    hipMemcpy(e->dfused, e->hfused.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess))
         return bail(fail(17, "plan upload failed"));
-    if (hipMemset(e->counter, 0, sizeof(unsigned)) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+    if (hipMemset(e->counter, 0, sizeof(unsigned) * kCounters) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
         return bail(fail(17, "counter initialisation failed"));
     e->prof_ms.assign(nsz * kNumSweeps, 0.0);
     e->prof_n.assign(nsz * kNumSweeps, 0);
@@ -1084,6 +1093,23 @@ int gs_step_geometry(const gs_engine* eng, int k, int* out) {
     out[5] = eng->fusable[size_t(k)] ? 1 << f.lg_grid : 0;
     out[6] = eng->fusable[size_t(k)] ? f.route2 : 0;
     out[7] = s.lg_iter;
+    return 0;
+}
+
+// Cap the blocks of one sweep at 2^lg (1..14, default 11); tiles beyond the cap loop inside each block.  Rebuilds
+// and uploads the step descriptors (synchronous).
+int gs_set_grid_log2(gs_engine* eng, int lg) {
+    if (!eng) return fail(16, "null engine");
+    if (lg < 1 || lg > kMaxGridLog2) return fail(16, "grid log2 must be in 1.." + std::to_string(kMaxGridLog2));
+    HIP_TRY(hipSetDevice(eng->device));
+    HIP_TRY(hipStreamSynchronize(eng->stream));
+    eng->grid_cap = lg;
+    if (int rc = build_steps(eng)) return rc;
+    const size_t ns = size_t(eng->plan.ns);
+    if (ns) {
+        HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+    }
     return 0;
 }
 

@@ -55,55 +55,86 @@ struct Cx<double2> {
     static __device__ __forceinline__ double2 narrow(double2 a) { return a; }
 };
 
+// Arena accesses carry the non-temporal hint: every sweep streams the table once, so its lines should not displace
+// anything in L2, and written lines leave few dirty lines behind.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float2 ld_nt(const float2* p) {
+    const f2v t = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+    return make_float2(t.x, t.y);
+}
+__device__ __forceinline__ void st_nt(float2* p, float2 v) {
+    const f2v t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<f2v*>(p));
+}
+__device__ __forceinline__ float4 ld_nt4(const float2* p) {
+    const f4v t = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+    return make_float4(t.x, t.y, t.z, t.w);
+}
+__device__ __forceinline__ void st_nt4(float2* p, float2 a, float2 b) {
+    const f4v t = {a.x, a.y, b.x, b.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<f4v*>(p));
+}
+__device__ __forceinline__ double2 ld_nt(const double2* p) {
+    const d2v t = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+    return make_double2(t.x, t.y);
+}
+__device__ __forceinline__ void st_nt(double2* p, double2 v) {
+    const d2v t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<d2v*>(p));
+}
+
 // VR consecutive arena entries at an aligned index, as 16-byte vector accesses.
 template <class S, int VR>
 struct Run;
 
 template <>
 struct Run<float2, 1> {
-    static __device__ __forceinline__ void load(const float2* m, uint64_t a, float2 (&e)[1]) { e[0] = m[a]; }
-    static __device__ __forceinline__ void store(float2* m, uint64_t a, const float2 (&e)[1]) { m[a] = e[0]; }
+    static __device__ __forceinline__ void load(const float2* m, uint64_t a, float2 (&e)[1]) { e[0] = ld_nt(m + a); }
+    static __device__ __forceinline__ void store(float2* m, uint64_t a, const float2 (&e)[1]) { st_nt(m + a, e[0]); }
 };
 template <>
 struct Run<float2, 2> {
     static __device__ __forceinline__ void load(const float2* m, uint64_t a, float2 (&e)[2]) {
-        const float4 x = *reinterpret_cast<const float4*>(m + a);
+        const float4 x = ld_nt4(m + a);
         e[0] = make_float2(x.x, x.y);
         e[1] = make_float2(x.z, x.w);
     }
     static __device__ __forceinline__ void store(float2* m, uint64_t a, const float2 (&e)[2]) {
-        *reinterpret_cast<float4*>(m + a) = make_float4(e[0].x, e[0].y, e[1].x, e[1].y);
+        st_nt4(m + a, e[0], e[1]);
     }
 };
 template <>
 struct Run<float2, 4> {
     static __device__ __forceinline__ void load(const float2* m, uint64_t a, float2 (&e)[4]) {
-        const float4 x = *reinterpret_cast<const float4*>(m + a);
-        const float4 y = *reinterpret_cast<const float4*>(m + a + 2);
+        const float4 x = ld_nt4(m + a);
+        const float4 y = ld_nt4(m + a + 2);
         e[0] = make_float2(x.x, x.y);
         e[1] = make_float2(x.z, x.w);
         e[2] = make_float2(y.x, y.y);
         e[3] = make_float2(y.z, y.w);
     }
     static __device__ __forceinline__ void store(float2* m, uint64_t a, const float2 (&e)[4]) {
-        *reinterpret_cast<float4*>(m + a) = make_float4(e[0].x, e[0].y, e[1].x, e[1].y);
-        *reinterpret_cast<float4*>(m + a + 2) = make_float4(e[2].x, e[2].y, e[3].x, e[3].y);
+        st_nt4(m + a, e[0], e[1]);
+        st_nt4(m + a + 2, e[2], e[3]);
     }
 };
 template <>
 struct Run<double2, 1> {
-    static __device__ __forceinline__ void load(const double2* m, uint64_t a, double2 (&e)[1]) { e[0] = m[a]; }
-    static __device__ __forceinline__ void store(double2* m, uint64_t a, const double2 (&e)[1]) { m[a] = e[0]; }
+    static __device__ __forceinline__ void load(const double2* m, uint64_t a, double2 (&e)[1]) { e[0] = ld_nt(m + a); }
+    static __device__ __forceinline__ void store(double2* m, uint64_t a, const double2 (&e)[1]) { st_nt(m + a, e[0]); }
 };
 template <>
 struct Run<double2, 2> {
     static __device__ __forceinline__ void load(const double2* m, uint64_t a, double2 (&e)[2]) {
-        e[0] = m[a];
-        e[1] = m[a + 1];
+        e[0] = ld_nt(m + a);
+        e[1] = ld_nt(m + a + 1);
     }
     static __device__ __forceinline__ void store(double2* m, uint64_t a, const double2 (&e)[2]) {
-        m[a] = e[0];
-        m[a + 1] = e[1];
+        st_nt(m + a, e[0]);
+        st_nt(m + a + 1, e[1]);
     }
 };
 
@@ -284,44 +315,11 @@ __device__ void decide(Ctl& c, const Epi& e, int k, int norm, const double* red)
     c.w[3] = m1.y * sc;
 }
 
-// Last block of a sweep: reduce the nq partial sums of all blocks in a fixed order, then run the chain.
+// Block-wide sum of v[NQ] over the 256 threads in a fixed tree order; the result is in sh[0][*].
 template <int NQ>
-__device__ void sweep_epilogue(double (&acc)[NQ], double* part, Ctl* ctl, const Epi& e, int sweep) {
-    // block sum (wave butterfly, then the 4 waves in order)
+__device__ __forceinline__ void block_tree_sum(double (&sh)[kThreads][NQ], const double (&v)[NQ], int t) {
 #pragma unroll
-    for (int q = 0; q < NQ; q++)
-        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
-    __shared__ double sh[kThreads][NQ];
-    __shared__ int s_last;
-    const int t = threadIdx.x;
-    if ((t & 63) == 0)
-#pragma unroll
-        for (int q = 0; q < NQ; q++) sh[t >> 6][q] = acc[q];
-    __syncthreads();
-    if (t == 0) {
-        double* mine = part + size_t(blockIdx.x) * kPartStride;
-#pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            double v = sh[0][q];
-#pragma unroll
-            for (int w = 1; w < kThreads / 64; w++) v += sh[w][q];
-            __hip_atomic_store(mine + q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        const unsigned done = __hip_atomic_fetch_add(e.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = done == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
-    double r[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; q++) r[q] = 0;
-    for (unsigned i = t; i < gridDim.x; i += kThreads)
-#pragma unroll
-        for (int q = 0; q < NQ; q++)
-            r[q] += __hip_atomic_load(part + size_t(i) * kPartStride + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-    for (int q = 0; q < NQ; q++) sh[t][q] = r[q];
+    for (int q = 0; q < NQ; q++) sh[t][q] = v[q];
     __syncthreads();
     for (int w = kThreads / 2; w > 0; w >>= 1) {
         if (t < w)
@@ -329,10 +327,11 @@ __device__ void sweep_epilogue(double (&acc)[NQ], double* part, Ctl* ctl, const 
             for (int q = 0; q < NQ; q++) sh[t][q] += sh[t + w][q];
         __syncthreads();
     }
-    if (t != 0) return;
-    double red[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; q++) red[q] = sh[0][q];
+}
+
+// Chain logic after a sweep, from its reduced sums (one thread).
+template <int NQ>
+__device__ void chain_step(const double (&red)[NQ], Ctl* ctl, const Epi& e, int sweep) {
     Ctl& c = *ctl;
     if (e.role == kRoleNorm) {
         decide(c, e, e.k, sweep, red);
@@ -345,7 +344,84 @@ __device__ void sweep_epilogue(double (&acc)[NQ], double* part, Ctl* ctl, const 
             else if (e.next == kNextFusedDirect) decide(c, e, e.k + 1, kDirect, red + 2);
         }
     }
+}
+
+// Fixed-order sum of the first NQ values of nblk block partials by one block; the result is in sh[0][*].
+template <int NQ>
+__device__ __forceinline__ void reduce_partials(double (&sh)[kThreads][NQ], const double* part, unsigned nblk, int t,
+                                                bool atomic_loads) {
+    double r[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) r[q] = 0;
+    for (unsigned i = t; i < nblk; i += kThreads)
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            r[q] += atomic_loads ? __hip_atomic_load(part + size_t(i) * kPartStride + q, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)
+                                 : part[size_t(i) * kPartStride + q];
+    block_tree_sum<NQ>(sh, r, t);
+}
+
+// End of a sweep block: the block's sums go to its partial slot.  Grids of at most kFenceBlocks blocks finish in
+// the kernel: the last block to arrive (agent-scope counter) reduces and runs the chain.  Larger grids end here and
+// sweep_reduce_kernel follows on the stream: an agent-scope release per block writes back the XCD's L2 each time
+// (tens of ns per block), which a launch boundary does once.
+template <int NQ>
+__device__ void sweep_epilogue(double (&acc)[NQ], double* part, Ctl* ctl, const Epi& e, int sweep) {
+    // block sum (wave butterfly, then the 4 waves in order)
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
+    __shared__ double sh[kThreads][NQ];
+    __shared__ int s_last;
+    const int t = threadIdx.x;
+    const unsigned nblk = gridDim.x;
+    const bool inkernel = nblk <= unsigned(kFenceBlocks);
+    if ((t & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) sh[t >> 6][q] = acc[q];
+    __syncthreads();
+    if (t == 0) {
+        double* mine = part + size_t(blockIdx.x) * kPartStride;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            double v = sh[0][q];
+#pragma unroll
+            for (int w = 1; w < kThreads / 64; w++) v += sh[w][q];
+            if (inkernel) __hip_atomic_store(mine + q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else mine[q] = v;
+        }
+        if (inkernel) {
+            const unsigned done = __hip_atomic_fetch_add(e.counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            s_last = done == nblk - 1;
+        }
+    }
+    if (!inkernel) return;
+    __syncthreads();
+    if (!s_last) return;
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    reduce_partials<NQ>(sh, part, nblk, t, true);
+    if (t != 0) return;
+    double red[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) red[q] = sh[0][q];
+    chain_step<NQ>(red, ctl, e, sweep);
     __hip_atomic_store(e.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Reduction and chain logic of a sweep of nblk > kFenceBlocks blocks (one block, launched after the sweep).
+template <int NQ>
+__global__ __launch_bounds__(kThreads) void sweep_reduce_kernel(const double* __restrict__ part, unsigned nblk,
+                                                                Ctl* __restrict__ ctl, Epi e, int sweep) {
+    if (ctl->dead) return;
+    __shared__ double sh[kThreads][NQ];
+    const int t = threadIdx.x;
+    reduce_partials<NQ>(sh, part, nblk, t, false);
+    if (t != 0) return;
+    double red[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) red[q] = sh[0][q];
+    chain_step<NQ>(red, ctl, e, sweep);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -613,6 +689,17 @@ int plain_lr(int storage) { return storage == 0 ? 2 : 1; }
 int fused_lr(int storage) { return storage == 0 ? 1 : 0; }
 
 void* decide_kernel_ptr() { return reinterpret_cast<void*>(&decide_kernel); }
+// Reduce kernel of a sweep kind (number of partial sums: Gram 4, direct 2, write 2, write + Gram 6, write + direct 4).
+void* reduce_kernel_ptr(int sweep) {
+    switch (sweep) {
+        case kGram: return reinterpret_cast<void*>(&sweep_reduce_kernel<4>);
+        case kDirect: return reinterpret_cast<void*>(&sweep_reduce_kernel<2>);
+        case kWrite: return reinterpret_cast<void*>(&sweep_reduce_kernel<2>);
+        case kWriteGram: return reinterpret_cast<void*>(&sweep_reduce_kernel<6>);
+        case kWriteDirect: return reinterpret_cast<void*>(&sweep_reduce_kernel<4>);
+    }
+    return nullptr;
+}
 void* publish_kernel_ptr() { return reinterpret_cast<void*>(&publish_kernel); }
 
 }  // namespace gs

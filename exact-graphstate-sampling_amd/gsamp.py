@@ -5,6 +5,7 @@ strings are numpy int8 arrays in STEP order ([nstr, ns]; -1 = '?').  There is no
 engine without the built library or without a HIP device raises.
 """
 import ctypes
+import hashlib
 import os
 
 import numpy as np
@@ -19,7 +20,8 @@ SEED_PLAIN, SEED_SEQ = 0, 1
 # every symbol include/gsamp_amd.h declares
 ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy", "gs_engine_create",
                "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
-               "gs_last_device_ms", "gs_last_error", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
+               "gs_last_device_ms", "gs_last_error", "gs_profile_enable", "gs_profile_read",
+               "gs_step_geometry", "gs_set_fusion", "gs_set_grid_log2", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
                "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms")
 
@@ -67,6 +69,7 @@ def library():
         "gs_profile_read": (I, [P, dp, ctypes.POINTER(I64)]),
         "gs_step_geometry": (I, [P, I, ip]),
         "gs_set_fusion": (I, [P, I]),
+        "gs_set_grid_log2": (I, [P, I]),
         "gs_copy_probe": (I, [I, I64, dp]),
         "gs_batch_create": (I, [ctypes.POINTER(P), ip, I, I, I, pp]),
         "gs_batch_set_matrices": (I, [P, I, dp]),
@@ -118,8 +121,11 @@ class Plan:
             raise ValueError("give exactly one of path, text")
         if path is not None:
             _check(lib.gs_plan_load(os.fsencode(path), ctypes.byref(self._h)))
+            with open(path, "rb") as f:
+                self.digest = hashlib.sha1(f.read()).hexdigest()
         else:
             _check(lib.gs_plan_parse(text.encode(), ctypes.byref(self._h)))
+            self.digest = hashlib.sha1(text.encode()).hexdigest()
         n, ns, R = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         lib.gs_plan_info(self._h, ctypes.byref(n), ctypes.byref(ns), ctypes.byref(R))
         self.n, self.ns, self.R = n.value, ns.value, R.value
@@ -255,6 +261,10 @@ class Engine:
     def set_fusion(self, on):
         """Fold each sampled step's norm pass into the previous write sweep where possible (default on)."""
         _check(library().gs_set_fusion(self._h, 1 if on else 0))
+
+    def set_grid_log2(self, lg):
+        """Cap every sweep at 2^lg blocks (1..14, default 11); tiles beyond the cap loop inside each block."""
+        _check(library().gs_set_grid_log2(self._h, int(lg)))
 
 
 class BatchEngine:

@@ -111,6 +111,21 @@ const char* gs_batch_last_error(void);
 /* Timing of the device work of the last call, in milliseconds (HIP events around the step kernels). */
 double gs_last_device_ms(const gs_engine* eng);
 
+/* ---- measurement and tuning (no counterpart in the reference; results do not depend on them). */
+/* Per-sweep timing: enable/disable, then read ms[ns][5] and launches[ns][5] accumulated since enabling
+ * (sweeps: 0 Gram norms, 1 direct norms, 2 write, 3 write + next Gram, 4 write + next direct). */
+int gs_profile_enable(gs_engine* eng, int on);
+int gs_profile_read(const gs_engine* eng, double* ms, int64_t* launches);
+/* Launch geometry of step k, out[8]: log2 cosets per 16-byte run, blocks, slots per coset, route, fusable, blocks
+ * of the fused sweep, next step's route, log2 tiles per block. */
+int gs_step_geometry(const gs_engine* eng, int k, int* out);
+/* Fold the next step's norm pass into the previous write (on, default) or run it as its own sweep (off). */
+int gs_set_fusion(gs_engine* eng, int on);
+/* Cap one sweep at 2^lg blocks (1..14, default 11); tiles beyond the cap loop inside each block. */
+int gs_set_grid_log2(gs_engine* eng, int lg);
+/* One plain device copy of `bytes` (read + write), *ms its duration: calibrates hardware byte counters. */
+int gs_copy_probe(int device, int64_t bytes, double* ms);
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char* gs_last_error(void);
 

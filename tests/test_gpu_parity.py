@@ -214,3 +214,31 @@ def test_plan_errors_from_engine(golden):
     with pytest.raises(gs.GsError) as ex:
         gs.Plan(text=" ".join(toks[:3] + toks[3:5]))
     assert ex.value.code == 3
+
+
+@pytest.fixture(scope="module")
+def c4_plan(tmp_path_factory):
+    path = str(tmp_path_factory.mktemp("c4") / "plan.txt")
+    egs.instances.config_instance("c4").write_plan(path)
+    return path
+
+
+@pytest.mark.parametrize("grid_log2", [14, 9, 6, 4])
+def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2):
+    """n=60 band-24 instance (2^20 arena, up to 2^10 blocks per sweep): samples from the FP64 arena equal the
+    oracle's bits and log2 Pr (1e-12) whatever the block cap -- one tile per block with a separate reduce kernel,
+    512 looping blocks with the reduce kernel, 64 or 16 looping blocks reducing in their last block."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    import subprocess
+    out = subprocess.run([os.path.join(root, "oracle", "gs_oracle"), "--c128", c4_plan, "sample", "3", "777"],
+                         capture_output=True, text=True, check=True).stdout.split("\n")
+    ref = [l.split() for l in out if l.strip()]
+    print(f"c4 sample 3 777 grid_log2={grid_log2}", flush=True)
+    with gs.Engine(c4_plan, gs.C128) as e:
+        e.set_grid_log2(grid_log2)
+        assert max(e.geometry(k)["grid"] for k in range(e.ns)) <= 1 << grid_log2
+        bits, lp = e.sample(3, 777)
+        lpp = e.prob(bits)
+    assert [egs.planio.format_bits(b) for b in bits] == [r[0] for r in ref]
+    assert close(lp, [float(r[1]) for r in ref], 0) < 1e-12
+    assert close(lpp, lp, 0) < 1e-12
