@@ -16,7 +16,7 @@ def _load():
     sys.modules["egs"] = mod
     spec.loader.exec_module(mod)
     for name in ("tableau", "graphform", "amp", "plan", "inplace", "engine", "gadgets", "instances", "planio", "gsamp",
-                 "adjoint", "autotune"):
+                 "adjoint", "autotune", "sliced", "doped"):
         importlib.import_module(f"egs.{name}")
     return mod
 

@@ -242,3 +242,32 @@ def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2):
     assert [egs.planio.format_bits(b) for b in bits] == [r[0] for r in ref]
     assert close(lp, [float(r[1]) for r in ref], 0) < 1e-12
     assert close(lpp, lp, 0) < 1e-12
+
+
+@pytest.mark.parametrize("storage", [gs.C64, gs.C128])
+def test_sliced_prefix_resume_with_twist(tmp_path, storage):
+    """Two slice vertices with suffix neighbours on the n=24 band-10 instance: the four slice tables of step K=18
+    (gs_prefix on each slice plan), summed by gs_resume with the twist, give the unsliced log2 Pr (gs_prob)."""
+    inst = egs.instances.Instance(24, 10, 1)
+    full = str(tmp_path / "full.txt")
+    inst.write_plan(full)
+    info = egs.sliced.make_sliced_plans(inst.am.A, inst.am.M, inst.order, [13, 15], str(tmp_path / "sl"), K=18)
+    assert info["twist"]
+    bits = np.random.default_rng(7).integers(0, 2, size=(6, 24)).astype(np.int8)
+    print(f"sliced [13, 15] K=18 storage={TAG[storage]}", flush=True)
+    with gs.Engine(full, storage) as e:
+        want = e.prob(bits)
+    cs = [c for c, _ in info["twists"]]
+    ts = [t for _, t in info["twists"]]
+    got = []
+    for b in bits:
+        tabs, lss = [], []
+        for p in info["plans"]:
+            with gs.Engine(p, storage) as e:
+                tab, nk, ntq, ls, _ = e.prefix(b, 18)
+            tabs.append(tab)
+            lss.append(ls)
+        with gs.Engine(info["plans"][0], storage) as e:
+            _, lpf, _, _ = e.resume(b, 18, tabs, lss, 5, 0, cs, ts)
+        got.append(lpf)
+    assert close(got, want, 0) < TOL[storage] * 10
