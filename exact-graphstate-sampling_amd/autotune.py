@@ -20,7 +20,8 @@ import time
 
 import numpy as np
 
-CANDIDATES = (12, 14, 16, 18)
+CANDIDATES = (8, 10, 12, 14)
+UNROLLS = (0, 1, 2)
 
 
 def _key(bt):
@@ -58,16 +59,18 @@ def step_times(bt, bits):
     return out
 
 
-def tune_tiles(bt, bits, candidates=CANDIDATES, cache=None, repeats=2):
-    """Pick lg per step index (the same for every instance, clamped to each step's coset count), apply it and return
-    ``dict(lg=[...], ms_before=..., ms_after=..., cached=bool)``."""
+def tune_tiles(bt, bits, candidates=CANDIDATES, unrolls=UNROLLS, cache=None, repeats=2):
+    """Pick lg per step index (the same for every instance, clamped to each step's coset count) and then the unroll
+    depth, apply them and return ``dict(lg=[...], lu=..., cached=bool, ms_default=..., ms_tuned=...)``."""
     table = _load(cache)
     key = _key(bt)
     if key in table:
-        lgs = table[key]
-        _apply(bt, lgs)
-        return dict(lg=lgs, cached=True)
+        ent = table[key]
+        _apply(bt, ent["lg"])
+        bt.set_unroll(ent["lu"])
+        return dict(lg=ent["lg"], lu=ent["lu"], cached=True)
     times = {}
+    bt.set_unroll(0)
     for lg in candidates:
         _apply(bt, [lg] * bt.maxns)
         t = np.full(bt.maxns, np.inf)
@@ -76,10 +79,16 @@ def tune_tiles(bt, bits, candidates=CANDIDATES, cache=None, repeats=2):
         times[lg] = t
     best = [int(min(candidates, key=lambda c: times[c][k])) for k in range(bt.maxns)]
     _apply(bt, best)
-    table[key] = best
+    tot = {}
+    for lu in unrolls:
+        bt.set_unroll(lu)
+        tot[lu] = min(float(np.sum(step_times(bt, bits))) for _ in range(repeats))
+    lu = min(unrolls, key=lambda u: tot[u])
+    bt.set_unroll(lu)
+    table[key] = {"lg": best, "lu": int(lu)}
     _store(cache, table)
-    return dict(lg=best, cached=False, ms_default=float(np.sum(times[candidates[0]])),
-                ms_tuned=float(sum(times[best[k]][k] for k in range(bt.maxns))))
+    return dict(lg=best, lu=int(lu), cached=False, ms_default=float(np.sum(times[12])) if 12 in times else None,
+                ms_tuned=tot[lu])
 
 
 def _apply(bt, lgs):

@@ -70,7 +70,6 @@ struct BInst {           // device view of one instance
     double* amp;         // [B][2]
     double* grad;        // [B][ns][2][2]: G_k(x) re, im (gradient mode)
     double* part;        // [groups][max items per group][64][4]
-    unsigned* counter;   // [groups]
     int B, ns, pad0, pad1;
 };
 
@@ -81,18 +80,40 @@ struct BLaunch {
     void* const* copy;   // [ninst] destination of a compact copy of this step's input table, or nullptr entries
 };
 
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+// Arena entries: raw (non-temporal) loads and stores of the storage type, widening to FP64 where used.
 template <class S>
 struct Sto;
 template <>
 struct Sto<float2> {
-    static __device__ __forceinline__ double2 ld(const float2* p) { const float2 v = *p; return make_double2(v.x, v.y); }
-    static __device__ __forceinline__ void st(float2* p, double2 v) { *p = make_float2(float(v.x), float(v.y)); }
+    static __device__ __forceinline__ float2 ld(const float2* p) {
+        const f2v t = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+        return make_float2(t.x, t.y);
+    }
+    static __device__ __forceinline__ void st(float2* p, float2 v) {
+        const f2v t = {v.x, v.y};
+        __builtin_nontemporal_store(t, reinterpret_cast<f2v*>(p));
+    }
+    static __device__ __forceinline__ double2 wide(float2 v) { return make_double2(v.x, v.y); }
+    static __device__ __forceinline__ float2 narrow(double2 v) { return make_float2(float(v.x), float(v.y)); }
 };
 template <>
 struct Sto<double2> {
-    static __device__ __forceinline__ double2 ld(const double2* p) { return *p; }
-    static __device__ __forceinline__ void st(double2* p, double2 v) { *p = v; }
+    static __device__ __forceinline__ double2 ld(const double2* p) {
+        const d2v t = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+        return make_double2(t.x, t.y);
+    }
+    static __device__ __forceinline__ void st(double2* p, double2 v) {
+        const d2v t = {v.x, v.y};
+        __builtin_nontemporal_store(t, reinterpret_cast<d2v*>(p));
+    }
+    static __device__ __forceinline__ double2 wide(double2 v) { return v; }
+    static __device__ __forceinline__ double2 narrow(double2 v) { return v; }
 };
+// Kernels are instantiated per storage S, NQ (the most slots per coset among the instances' step k: 1, 2 or 4) and
+// LU (log2 cosets a lane gathers at once: 2^LU * NQ loads in flight per lane).
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 cneg(double2 a) { return make_double2(-a.x, -a.y); }
@@ -104,74 +125,99 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double norm2(double2 a) { return fma(a.x, a.x, a.y * a.y); }
 
-// Coset range of this wave: Gray-ordered walk over 2^(lg - lw) cosets, base and sign parity kept incrementally.
+// Coset range of this wave: 2^lsub cosets c = cstart | (gray(t) << lu) | u, t < 2^(lsub - lu), u < 2^lu with
+// lu = min(LU, lsub): binary inside a group of 2^lu cosets, whose bases are base(t) ^ ubase[u] (all gathered at once),
+// Gray-ordered across groups, so the next group's base is one XOR away.
+template <int LU>
 struct Walk {
-    uint64_t cstart, base;
-    int pc, lsub;
+    static constexpr int U = 1 << LU;
+    uint64_t cstart, base;    // base of cstart | (gray(t) << lu)
+    int pc, lsub, lu;
     bool on;
-    __device__ Walk(const BItem& it, const BStep& s, int wave) {
+    uint64_t ubase[U];
+    int upc[U];
+    __device__ Walk(const BItem& it, const uint64_t* __restrict__ cols, uint64_t rhoc, int nk, int wave) {
         const int lw = it.lg >= 2 ? 2 : it.lg;
         on = wave < (1 << lw);
         lsub = it.lg - lw;
         cstart = it.c0 | (uint64_t(wave) << lsub);
         base = 0;
         pc = 0;
-        for (int j = 0; j < s.nk; j++)
+        for (int j = lsub; j < nk; j++)
             if ((cstart >> j) & 1) {
-                base ^= s.cols[j];
-                pc ^= int((s.rhoc >> j) & 1);
+                base ^= cols[j];
+                pc ^= int((rhoc >> j) & 1);
             }
+        lu = lsub < LU ? lsub : LU;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            uint64_t a = 0;
+            int p = 0;
+#pragma unroll
+            for (int j = 0; j < LU; j++)
+                if (j < lu && ((u >> j) & 1)) {
+                    a ^= cols[j];
+                    p ^= int((rhoc >> j) & 1);
+                }
+            ubase[u] = a;
+            upc[u] = p;
+        }
     }
-    __device__ uint64_t count() const { return on ? uint64_t(1) << lsub : 0; }
-    __device__ void step(const BStep& s, uint64_t i) {        // move to the i-th coset (i >= 1)
-        const int j = __builtin_ctzll(i);
-        base ^= s.cols[j];
-        pc ^= int((s.rhoc >> j) & 1);
+    __device__ uint64_t groups() const { return on ? uint64_t(1) << (lsub - lu) : 0; }
+    __device__ bool valid(int u) const { return u < (1 << lu); }
+    __device__ void next(const uint64_t* __restrict__ cols, uint64_t rhoc, uint64_t t) {   // to group t >= 1
+        const int j = lu + __builtin_ctzll(t);
+        base ^= cols[j];
+        pc ^= int((rhoc >> j) & 1);
     }
-    __device__ uint64_t coset(uint64_t i) const { return cstart | (i ^ (i >> 1)); }
+    __device__ uint64_t coset(uint64_t t, int u) const { return cstart | ((t ^ (t >> 1)) << lu) | uint64_t(u); }
 };
 
-// Per-lane sum of NQ values over the block's waves, then over the items of (inst, group): returns true in the threads
-// of the last item's wave 0, with tot[] the fixed-order total of lane `lane`.
+// Per-lane sum of NQ values over the block's waves (fixed order), stored as this item's partial (plain stores: the
+// per-(instance, lane group) reduction runs in batch_reduce_kernel after the step, at a launch boundary).
 template <int NQ>
-__device__ bool lane_reduce(const BInst& I, const BItem& it, int wave, int lane, const double (&v)[NQ], double (&tot)[NQ]) {
+__device__ void lane_store(const BInst& I, const BItem& it, int wave, int lane, const double (&v)[NQ]) {
     __shared__ double sh[kWaves][kLanes][NQ];
-    __shared__ int s_last;
 #pragma unroll
     for (int q = 0; q < NQ; q++) sh[wave][lane][q] = v[q];
     __syncthreads();
+    if (wave != 0) return;
     double* part = I.part + ((size_t(it.group) * size_t(it.count) + size_t(it.index)) * kLanes + lane) * 4;
-    if (wave == 0)
 #pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            double t = sh[0][lane][q];
-            for (int w = 1; w < kWaves; w++) t += sh[w][lane][q];
-            __hip_atomic_store(part + q, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned done = __hip_atomic_fetch_add(I.counter + it.group, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = done == unsigned(it.count) - 1;
+    for (int q = 0; q < NQ; q++) {
+        double t = sh[0][lane][q];
+        for (int w = 1; w < kWaves; w++) t += sh[w][lane][q];
+        part[q] = t;
     }
-    __syncthreads();
-    if (!s_last || wave != 0) return false;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
-#pragma unroll
-    for (int q = 0; q < NQ; q++) tot[q] = 0;
-    const double* gp = I.part + size_t(it.group) * size_t(it.count) * kLanes * 4;
-    for (int i = 0; i < it.count; i++)
-#pragma unroll
-        for (int q = 0; q < NQ; q++)
-            tot[q] += __hip_atomic_load(gp + (size_t(i) * kLanes + lane) * 4 + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (lane == 0) __hip_atomic_store(I.counter + it.group, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
 }
 
+// The fields of a step that the inner loops use, copied to registers once per block (read through a reference to
+// global memory, they would be reloaded after every arena store the compiler cannot prove disjoint from them).
+struct SR {
+    uint64_t off[4], g, rhoc;
+    int ntq, m;
+    int tau0[4], tau1[4], par[4];
+    __device__ explicit SR(const BStep& s) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            off[q] = s.off[q];
+            tau0[q] = s.tau0[q];
+            tau1[q] = s.tau1[q];
+            par[q] = s.par[q];
+        }
+        g = s.g;
+        rhoc = s.rhoc;
+        ntq = s.ntq;
+        m = s.m;
+    }
+};
+
 // Routed signed slot sums A[tau][x] of one coset from its ntq slot values.
-__device__ __forceinline__ void route(const BStep& s, const double2 (&v)[4], int pc, double2 (&A)[2][2]) {
+template <int NQ>
+__device__ __forceinline__ void route(const SR& s, const double2 (&v)[NQ], int pc, double2 (&A)[2][2]) {
     A[0][0] = A[0][1] = A[1][0] = A[1][1] = make_double2(0, 0);
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < NQ; q++) {
         if (q >= s.ntq) break;
         const double2 w = v[q];
         const double2 ws = (pc ^ s.par[q]) ? cneg(w) : w;
@@ -180,7 +226,7 @@ __device__ __forceinline__ void route(const BStep& s, const double2 (&v)[4], int
     }
 }
 
-template <class S>
+template <class S, int NQ, int LU>
 __global__ __launch_bounds__(kBThreads) void batch_step_kernel(const BInst* __restrict__ insts,
                                                                const BItem* __restrict__ items, BLaunch L) {
     const BItem it = items[blockIdx.x];
@@ -194,51 +240,54 @@ __global__ __launch_bounds__(kBThreads) void batch_step_kernel(const BInst* __re
     const int bv = I.bitsT[size_t(k) * B + b];
     const double2 w0 = make_double2(s.mre[bv][0], s.mim[bv][0]);
     const double2 w1 = make_double2(s.mre[bv][1], s.mim[bv][1]);
-    S* mem = static_cast<S*>(I.mem);
-    S* cp = L.copy ? static_cast<S*>(L.copy[it.inst]) : nullptr;
+    S* __restrict__ mem = static_cast<S*>(I.mem);
+    S* __restrict__ cp = L.copy ? static_cast<S*>(L.copy[it.inst]) : nullptr;
     double nb = 0;
-    Walk wk(it, s, wave);
-    const uint64_t n = wk.count();
-    for (uint64_t i = 0; i < n; i++) {
-        if (i) wk.step(s, i);
-        double2 v[4];
+    constexpr int U = 1 << LU;
+    const SR r(s);
+    Walk<LU> wk(it, s.cols, r.rhoc, s.nk, wave);
+    const uint64_t ng = wk.groups();
+    const int ntq = r.ntq;
+    for (uint64_t t = 0; t < ng; t++) {
+        if (t) wk.next(s.cols, r.rhoc, t);
+        S v[U][NQ];
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            if (q < s.ntq) v[q] = Sto<S>::ld(mem + (wk.base ^ s.off[q]) * B + b);
-        if (cp) {
-            const uint64_t c = wk.coset(i);
+        for (int u = 0; u < U; u++)                         // every load of the group first
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (q < s.ntq) Sto<S>::st(cp + (c * s.ntq + q) * B + b, v[q]);
-        }
-        double2 A[2][2];
-        route(s, v, wk.pc, A);
-        const double2 o0 = cfma(w0, A[0][0], cmul(w1, A[0][1]));
-        Sto<S>::st(mem + wk.base * B + b, o0);
-        nb += norm2(o0);
-        if (s.m == 2) {
-            const double2 o1 = cfma(w0, A[1][0], cmul(w1, A[1][1]));
-            Sto<S>::st(mem + (wk.base ^ s.g) * B + b, o1);
-            nb += norm2(o1);
+            for (int q = 0; q < NQ; q++)
+                if (q < ntq && wk.valid(u)) v[u][q] = Sto<S>::ld(mem + (wk.base ^ wk.ubase[u] ^ r.off[q]) * B + b);
+        if (cp)
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int q = 0; q < NQ; q++)
+                    if (q < ntq && wk.valid(u)) Sto<S>::st(cp + (wk.coset(t, u) * ntq + q) * B + b, v[u][q]);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (!wk.valid(u)) continue;
+            double2 vw[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) vw[q] = q < ntq ? Sto<S>::wide(v[u][q]) : make_double2(0, 0);
+            double2 A[2][2];
+            route(r, vw, wk.pc ^ wk.upc[u], A);
+            const uint64_t a0 = wk.base ^ wk.ubase[u];
+            const double2 o0 = cfma(w0, A[0][0], cmul(w1, A[0][1]));
+            Sto<S>::st(mem + a0 * B + b, Sto<S>::narrow(o0));
+            nb += norm2(o0);
+            if (r.m == 2) {
+                const double2 o1 = cfma(w0, A[1][0], cmul(w1, A[1][1]));
+                Sto<S>::st(mem + (a0 ^ r.g) * B + b, Sto<S>::narrow(o1));
+                nb += norm2(o1);
+            }
         }
     }
     if (!L.bookkeeping) return;
-    double tot[1];
     const double val[1] = {nb};
-    if (!lane_reduce<1>(I, it, wave, lane, val, tot)) return;
-    const int ns = I.ns;
-    if (!I.dead[b]) {                                       // chain1 bookkeeping of lane b (gsamp2.cpp:67)
-        I.cond[size_t(b) * ns + k] = tot[0] / (2 * I.prev[b]);
-        I.prev[b] = tot[0];
-        if (tot[0] <= 0) {
-            for (int q = k; q < ns; q++) I.cond[size_t(b) * ns + q] = 0;
-            I.dead[b] = 1;
-        }
-    }
+    lane_store<1>(I, it, wave, lane, val);
 }
 
 // Put a compact copy of step k's input table back into the forward arena (restart of a checkpoint segment).
-template <class S>
+template <class S, int NQ, int LU>
 __global__ __launch_bounds__(kBThreads) void batch_restore_kernel(const BInst* __restrict__ insts,
                                                                   const BItem* __restrict__ items, BLaunch L) {
     const BItem it = items[blockIdx.x];
@@ -248,23 +297,33 @@ __global__ __launch_bounds__(kBThreads) void batch_restore_kernel(const BInst* _
     const int wave = threadIdx.x / kLanes;
     const int B = I.B;
     const int b = it.group * kLanes + lane;
-    S* mem = static_cast<S*>(I.mem);
-    const S* cp = static_cast<const S*>(L.copy[it.inst]);
+    S* __restrict__ mem = static_cast<S*>(I.mem);
+    const S* __restrict__ cp = static_cast<const S*>(L.copy[it.inst]);
     if (!cp) return;
-    Walk wk(it, s, wave);
-    const uint64_t n = wk.count();
-    for (uint64_t i = 0; i < n; i++) {
-        if (i) wk.step(s, i);
-        const uint64_t c = wk.coset(i);
+    constexpr int U = 1 << LU;
+    const SR r(s);
+    Walk<LU> wk(it, s.cols, r.rhoc, s.nk, wave);
+    const uint64_t ng = wk.groups();
+    const int ntq = r.ntq;
+    for (uint64_t t = 0; t < ng; t++) {
+        if (t) wk.next(s.cols, r.rhoc, t);
+        S v[U][NQ];
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            if (q < s.ntq) mem[(wk.base ^ s.off[q]) * B + b] = cp[(c * s.ntq + q) * B + b];
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                if (q < ntq && wk.valid(u)) v[u][q] = Sto<S>::ld(cp + (wk.coset(t, u) * ntq + q) * B + b);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                if (q < ntq && wk.valid(u)) Sto<S>::st(mem + (wk.base ^ wk.ubase[u] ^ r.off[q]) * B + b, v[u][q]);
     }
 }
 
 // Backward step k: lambda_k (adjoint arena, at the coset's outputs) and the compact copy of P_{k-1} give
 // G_k(x) += sum_tau lambda[tau] A[tau][x]; lambda_{k-1} goes to the coset's input slots.
-template <class S>
+template <class S, int NQ, int LU>
 __global__ __launch_bounds__(kBThreads) void batch_back_kernel(const BInst* __restrict__ insts,
                                                                const BItem* __restrict__ items, BLaunch L) {
     const BItem it = items[blockIdx.x];
@@ -278,43 +337,88 @@ __global__ __launch_bounds__(kBThreads) void batch_back_kernel(const BInst* __re
     const int bv = I.bitsT[size_t(k) * B + b];
     const double2 w0 = make_double2(s.mre[bv][0], s.mim[bv][0]);
     const double2 w1 = make_double2(s.mre[bv][1], s.mim[bv][1]);
-    S* lam = static_cast<S*>(I.lam);
-    const S* cp = static_cast<const S*>(L.copy[it.inst]);
+    S* __restrict__ lam = static_cast<S*>(I.lam);
+    const S* __restrict__ cp = static_cast<const S*>(L.copy[it.inst]);
     double2 g0 = make_double2(0, 0), g1 = g0;
-    Walk wk(it, s, wave);
-    const uint64_t n = wk.count();
-    for (uint64_t i = 0; i < n; i++) {
-        if (i) wk.step(s, i);
-        const uint64_t c = wk.coset(i);
-        double2 v[4];
+    constexpr int U = 1 << LU;
+    const SR r(s);
+    Walk<LU> wk(it, s.cols, r.rhoc, s.nk, wave);
+    const uint64_t ng = wk.groups();
+    const int ntq = r.ntq;
+    for (uint64_t t = 0; t < ng; t++) {
+        if (t) wk.next(s.cols, r.rhoc, t);
+        S v[U][NQ], l0[U], l1[U];
 #pragma unroll
-        for (int q = 0; q < 4; q++)
-            if (q < s.ntq) v[q] = Sto<S>::ld(cp + (c * s.ntq + q) * B + b);
-        double2 A[2][2];
-        route(s, v, wk.pc, A);
-        const double2 l0 = Sto<S>::ld(lam + wk.base * B + b);
-        const double2 l1 = s.m == 2 ? Sto<S>::ld(lam + (wk.base ^ s.g) * B + b) : make_double2(0, 0);
-        g0 = cfma(l0, A[0][0], g0);
-        g1 = cfma(l0, A[0][1], g1);
-        if (s.m == 2) {
-            g0 = cfma(l1, A[1][0], g0);
-            g1 = cfma(l1, A[1][1], g1);
+        for (int u = 0; u < U; u++) {                       // every load of the group first
+            if (!wk.valid(u)) continue;
+            const uint64_t a0 = wk.base ^ wk.ubase[u];
+#pragma unroll
+            for (int q = 0; q < NQ; q++)
+                if (q < ntq) v[u][q] = Sto<S>::ld(cp + (wk.coset(t, u) * ntq + q) * B + b);
+            l0[u] = Sto<S>::ld(lam + a0 * B + b);
+            if (r.m == 2) l1[u] = Sto<S>::ld(lam + (a0 ^ r.g) * B + b);
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            if (q >= s.ntq) break;
-            const double2 lt0 = s.tau0[q] ? l1 : l0;
-            const double2 lt1 = s.tau1[q] ? l1 : l0;
-            const double2 x1 = cmul(w1, lt1);
-            const double2 out = cfma(w0, lt0, (wk.pc ^ s.par[q]) ? cneg(x1) : x1);
-            Sto<S>::st(lam + (wk.base ^ s.off[q]) * B + b, out);
+        for (int u = 0; u < U; u++) {
+            if (!wk.valid(u)) continue;
+            const uint64_t a0 = wk.base ^ wk.ubase[u];
+            const int pc = wk.pc ^ wk.upc[u];
+            double2 vw[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) vw[q] = q < ntq ? Sto<S>::wide(v[u][q]) : make_double2(0, 0);
+            double2 A[2][2];
+            route(r, vw, pc, A);
+            const double2 m0 = Sto<S>::wide(l0[u]);
+            const double2 m1 = r.m == 2 ? Sto<S>::wide(l1[u]) : make_double2(0, 0);
+            g0 = cfma(m0, A[0][0], g0);
+            g1 = cfma(m0, A[0][1], g1);
+            if (r.m == 2) {
+                g0 = cfma(m1, A[1][0], g0);
+                g1 = cfma(m1, A[1][1], g1);
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                if (q >= ntq) break;
+                const double2 lt0 = r.tau0[q] ? m1 : m0;
+                const double2 lt1 = r.tau1[q] ? m1 : m0;
+                const double2 x1 = cmul(w1, lt1);
+                const double2 out = cfma(w0, lt0, (pc ^ r.par[q]) ? cneg(x1) : x1);
+                Sto<S>::st(lam + (a0 ^ r.off[q]) * B + b, Sto<S>::narrow(out));
+            }
         }
     }
-    double tot[4];
     const double val[4] = {g0.x, g0.y, g1.x, g1.y};
-    if (!lane_reduce<4>(I, it, wave, lane, val, tot)) return;
-    double* G = I.grad + (size_t(b) * I.ns + k) * 4;
-    for (int q = 0; q < 4; q++) G[q] = tot[q];
+    lane_store<4>(I, it, wave, lane, val);
+}
+
+// After step k: per (instance, lane), the fixed-order sum of the items' partials (cnt[inst] items per lane group),
+// then chain1 bookkeeping (mode 0: cond, prev, dead; gsamp2.cpp:67) or the gradient entry G_k (mode 1).
+__global__ __launch_bounds__(kLanes) void batch_reduce_kernel(const BInst* __restrict__ insts,
+                                                              const int* __restrict__ cnt, int k, int mode) {
+    const int inst = blockIdx.y, group = blockIdx.x, lane = threadIdx.x;
+    const BInst& I = insts[inst];
+    const int count = cnt[inst];
+    if (count == 0 || group >= I.B / kLanes) return;
+    const int b = group * kLanes + lane;
+    const int nq = mode == 0 ? 1 : 4;
+    double tot[4] = {0, 0, 0, 0};
+    const double* gp = I.part + size_t(group) * size_t(count) * kLanes * 4;
+    for (int i = 0; i < count; i++)
+        for (int q = 0; q < nq; q++) tot[q] += gp[(size_t(i) * kLanes + lane) * 4 + q];
+    const int ns = I.ns;
+    if (mode == 1) {
+        double* G = I.grad + (size_t(b) * ns + k) * 4;
+        for (int q = 0; q < 4; q++) G[q] = tot[q];
+        return;
+    }
+    if (!I.dead[b]) {
+        I.cond[size_t(b) * ns + k] = tot[0] / (2 * I.prev[b]);
+        I.prev[b] = tot[0];
+        if (tot[0] <= 0) {
+            for (int q = k; q < ns; q++) I.cond[size_t(b) * ns + q] = 0;
+            I.dead[b] = 1;
+        }
+    }
 }
 
 template <class S>
@@ -324,7 +428,7 @@ __global__ void batch_init_kernel(const BInst* __restrict__ insts, int ninst) {
     const BInst& I = insts[i];
     S* mem = static_cast<S*>(I.mem);
     for (int b = threadIdx.x; b < I.B; b += blockDim.x) {
-        Sto<S>::st(mem + b, make_double2(1.0, 0.0));
+        Sto<S>::st(mem + b, Sto<S>::narrow(make_double2(1.0, 0.0)));
         I.prev[b] = 1.0;
         I.dead[b] = 0;
     }
@@ -338,7 +442,7 @@ __global__ void batch_finish_kernel(const BInst* __restrict__ insts, int ninst) 
     const S* mem = static_cast<const S*>(I.mem);
     for (int b = threadIdx.x; b < I.B; b += blockDim.x) {
         I.logp[b] = I.dead[b] ? -INFINITY : log2(I.prev[b]) - I.ns;
-        const double2 z = Sto<S>::ld(mem + b);
+        const double2 z = Sto<S>::wide(Sto<S>::ld(mem + b));
         I.amp[2 * b] = z.x;
         I.amp[2 * b + 1] = z.y;
     }
@@ -351,7 +455,7 @@ __global__ void batch_lambda_init_kernel(const BInst* __restrict__ insts, int ni
     if (i >= ninst) return;
     const BInst& I = insts[i];
     S* lam = static_cast<S*>(I.lam);
-    for (int b = threadIdx.x; b < I.B; b += blockDim.x) Sto<S>::st(lam + b, make_double2(1.0, 0.0));
+    for (int b = threadIdx.x; b < I.B; b += blockDim.x) Sto<S>::st(lam + b, Sto<S>::narrow(make_double2(1.0, 0.0)));
 }
 
 thread_local std::string g_berr;
@@ -383,7 +487,11 @@ struct gs_batch {
     std::vector<BItem*> ditems;
     std::vector<int> nitems;
     std::vector<std::vector<int>> lg;     // [inst][step]
-    int lg_default = 12;
+    int lg_default = 8;
+    std::vector<int> nq;                  // per step: most slots per coset among the instances (kernel variant)
+    int* dcnt = nullptr;                  // device [maxns][ninst]: items per lane group of instance i at step k
+    int maxgroups = 1;
+    int lu = 2;                           // log2 cosets a lane gathers at once
     // graph replay
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -414,8 +522,13 @@ T* dalloc(gs_batch* b, size_t n) {
 
 int build_items(gs_batch* bt) {
     for (auto* p : bt->ditems) (void)hipFree(p);
+    const size_t ni = bt->plans.size();
+    std::vector<int> cnt(size_t(bt->maxns) * ni, 0);
+    bt->maxgroups = 1;
+    for (size_t i = 0; i < ni; i++) bt->maxgroups = std::max(bt->maxgroups, bt->B[i] / kLanes);
     bt->ditems.assign(size_t(bt->maxns), nullptr);
     bt->nitems.assign(size_t(bt->maxns), 0);
+    bt->nq.assign(size_t(bt->maxns), 1);
     for (int k = 0; k < bt->maxns; k++) {
         std::vector<BItem> items;
         for (size_t i = 0; i < bt->plans.size(); i++) {
@@ -424,6 +537,7 @@ int build_items(gs_batch* bt) {
             const int nk = P.steps[size_t(k)].nk;
             const int lg = std::min(nk, bt->lg[i][size_t(k)]);
             const int count = 1 << (nk - lg);
+            cnt[size_t(k) * ni + i] = count;
             for (int g = 0; g < bt->B[i] / kLanes; g++)
                 for (int q = 0; q < count; q++) {
                     BItem it{};
@@ -436,6 +550,10 @@ int build_items(gs_batch* bt) {
                     items.push_back(it);
                 }
         }
+        int nqk = 1;
+        for (size_t i = 0; i < bt->plans.size(); i++)
+            if (k < bt->plans[i].ns) nqk = std::max(nqk, bt->plans[i].steps[size_t(k)].ntq);
+        bt->nq[size_t(k)] = nqk;
         bt->nitems[size_t(k)] = int(items.size());
         void* d = nullptr;
         if (hipMalloc(&d, std::max<size_t>(items.size() * sizeof(BItem), 16)) != hipSuccess)
@@ -444,6 +562,12 @@ int build_items(gs_batch* bt) {
             return bfail(17, "work-item upload failed");
         bt->ditems[size_t(k)] = static_cast<BItem*>(d);
     }
+    if (!bt->dcnt && hipMalloc(&bt->dcnt, std::max<size_t>(cnt.size() * sizeof(int), 16)) != hipSuccess) {
+        bt->dcnt = nullptr;
+        return bfail(11, "count table allocation failed");
+    }
+    if (hipMemcpy(bt->dcnt, cnt.data(), cnt.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+        return bfail(17, "count table upload failed");
     return 0;
 }
 
@@ -457,13 +581,56 @@ int max_items(const gs_batch* bt, size_t i) {
     return m;
 }
 
+// Instantiation of kernel K<S, NQ, LU> for NQ in {1, 2, 4}, LU in 0..3.
+#define GS_BATCH_VARIANT(K, S, nq, lu)                                                                            \
+    [&]() -> const void* {                                                                                         \
+        switch ((nq) * 4 + (lu)) {                                                                                 \
+            case 4: return reinterpret_cast<const void*>(&K<S, 1, 0>);                                             \
+            case 5: return reinterpret_cast<const void*>(&K<S, 1, 1>);                                             \
+            case 6: return reinterpret_cast<const void*>(&K<S, 1, 2>);                                             \
+            case 7: return reinterpret_cast<const void*>(&K<S, 1, 3>);                                             \
+            case 8: return reinterpret_cast<const void*>(&K<S, 2, 0>);                                             \
+            case 9: return reinterpret_cast<const void*>(&K<S, 2, 1>);                                             \
+            case 10: return reinterpret_cast<const void*>(&K<S, 2, 2>);                                            \
+            case 11: return reinterpret_cast<const void*>(&K<S, 2, 3>);                                            \
+            case 16: return reinterpret_cast<const void*>(&K<S, 4, 0>);                                            \
+            case 17: return reinterpret_cast<const void*>(&K<S, 4, 1>);                                            \
+            case 18: return reinterpret_cast<const void*>(&K<S, 4, 2>);                                            \
+            case 19: return reinterpret_cast<const void*>(&K<S, 4, 3>);                                            \
+        }                                                                                                          \
+        return nullptr;                                                                                            \
+    }()
+
+enum BKernel { kBStep, kBRestore, kBBack };
+
+template <class S>
+int launch_batch(gs_batch* bt, BKernel kind, int k, BLaunch L) {
+    const int n = bt->nitems[size_t(k)];
+    if (n == 0) return 0;
+    const int nq = bt->nq[size_t(k)], lu = bt->lu;
+    const void* fn = kind == kBStep ? GS_BATCH_VARIANT(batch_step_kernel, S, nq, lu)
+                   : kind == kBRestore ? GS_BATCH_VARIANT(batch_restore_kernel, S, nq, lu)
+                                       : GS_BATCH_VARIANT(batch_back_kernel, S, nq, lu);
+    if (!fn) return bfail(16, "no kernel variant");
+    const BInst* insts = bt->dinst;
+    const BItem* items = bt->ditems[size_t(k)];
+    void* args[] = {&insts, &items, &L};
+    return hipLaunchKernel(fn, dim3(n), dim3(kBThreads), args, 0, bt->stream) == hipSuccess ? 0
+                                                                                      : bfail(17, "step launch failed");
+}
+
+// Per-(instance, lane) reduction of step k's item partials: mode 0 chain bookkeeping, mode 1 gradient entry.
+int launch_reduce(gs_batch* bt, int k, int mode) {
+    if (bt->nitems[size_t(k)] == 0) return 0;
+    hipLaunchKernelGGL(batch_reduce_kernel, dim3(bt->maxgroups, int(bt->plans.size())), dim3(kLanes), 0, bt->stream,
+                       bt->dinst, bt->dcnt + size_t(k) * bt->plans.size(), k, mode);
+    return hipGetLastError() == hipSuccess ? 0 : bfail(17, "reduce launch failed");
+}
+
 template <class S>
 int launch_step(gs_batch* bt, int k, int bookkeeping, void* const* copy) {
-    if (bt->nitems[size_t(k)] == 0) return 0;
-    BLaunch L{k, bookkeeping, copy};
-    hipLaunchKernelGGL(batch_step_kernel<S>, dim3(bt->nitems[size_t(k)]), dim3(kBThreads), 0, bt->stream, bt->dinst,
-                       bt->ditems[size_t(k)], L);
-    return hipGetLastError() == hipSuccess ? 0 : bfail(17, "step launch failed");
+    if (int rc = launch_batch<S>(bt, kBStep, k, BLaunch{k, bookkeeping, copy})) return rc;
+    return bookkeeping ? launch_reduce(bt, k, 0) : 0;
 }
 
 template <class S>
@@ -593,6 +760,36 @@ size_t grad_bytes(const gs_batch* bt, int seg) {
     return tot * bt->esz;
 }
 
+// Steps per checkpoint segment: among the segment lengths whose copies fit the budget (mem_budget <= 0: 70 % of the
+// device memory free now, plus what the current copies hold), the one that re-sweeps the fewest entries (every
+// segment but the last is recomputed once); the smallest footprint when none fits.
+int choose_segment(const gs_batch* bt, double mem_budget) {
+    if (mem_budget <= 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+        mem_budget = 0.7 * double(fr) + (bt->seg ? double(grad_bytes(bt, bt->seg)) : 0.0);
+    }
+    int best = -1, smallest = 1;
+    double best_cost = 0, small_bytes = -1;
+    for (int sg = 1; sg <= bt->maxns; sg++) {
+        const double bytes = double(grad_bytes(bt, sg));
+        if (small_bytes < 0 || bytes < small_bytes) {
+            small_bytes = bytes;
+            smallest = sg;
+        }
+        if (bytes > mem_budget) continue;
+        const int last_start = ((bt->maxns - 1) / sg) * sg;
+        double cost = 0;
+        for (size_t i = 0; i < bt->plans.size(); i++)
+            for (int k = 0; k < std::min(bt->plans[i].ns, last_start); k++) cost += double(copy_entries(bt, i, k));
+        if (best < 0 || cost <= best_cost) {
+            best = sg;
+            best_cost = cost;
+        }
+    }
+    return best < 0 ? smallest : best;
+}
+
 template <class S>
 int grad_t(gs_batch* bt) {
     const int ninst = int(bt->plans.size());
@@ -628,19 +825,13 @@ int grad_t(gs_batch* bt) {
     for (int s = last; s >= 0; s--) {
         const int k0 = s * seg, k1 = std::min(ns, k0 + seg);
         if (s != last) {                                    // recompute the segment from its checkpoint
-            if (bt->nitems[size_t(k0)]) {
-                BLaunch L{k0, 0, ck_ptr(s)};
-                hipLaunchKernelGGL(batch_restore_kernel<S>, dim3(bt->nitems[size_t(k0)]), dim3(kBThreads), 0, bt->stream,
-                                   bt->dinst, bt->ditems[size_t(k0)], L);
-            }
+            if (int rc = launch_batch<S>(bt, kBRestore, k0, BLaunch{k0, 0, ck_ptr(s)})) return rc;
             for (int k = k0; k < k1; k++)
                 if (int rc = launch_step<S>(bt, k, 0, rec_ptr(k - k0))) return rc;
         }
         for (int k = k1 - 1; k >= k0; k--) {
-            if (!bt->nitems[size_t(k)]) continue;
-            BLaunch L{k, 0, rec_ptr(k - k0)};
-            hipLaunchKernelGGL(batch_back_kernel<S>, dim3(bt->nitems[size_t(k)]), dim3(kBThreads), 0, bt->stream,
-                               bt->dinst, bt->ditems[size_t(k)], L);
+            if (int rc = launch_batch<S>(bt, kBBack, k, BLaunch{k, 0, rec_ptr(k - k0)})) return rc;
+            if (int rc = launch_reduce(bt, k, 1)) return rc;
         }
     }
     (void)hipEventRecord(e1, bt->stream);
@@ -670,6 +861,7 @@ int gs_batch_create(const gs_plan* const* plans, const int* lanes, int nplans, i
     bt->device = device;
     bt->storage = storage;
     bt->esz = storage == GS_C64 ? 8 : 16;
+    bt->lu = storage == GS_C64 ? 2 : 1;
     auto bail = [&](int rc) {
         for (void* p : bt->allocs) (void)hipFree(p);
         for (auto* p : bt->ditems) (void)hipFree(p);
@@ -734,11 +926,8 @@ int gs_batch_create(const gs_plan* const* plans, const int* lanes, int nplans, i
         I.logp = dalloc<double>(bt, size_t(B));
         I.amp = dalloc<double>(bt, size_t(B) * 2);
         I.part = dalloc<double>(bt, size_t(B / kLanes) * size_t(max_items(bt, size_t(i))) * kLanes * 4);
-        I.counter = dalloc<unsigned>(bt, size_t(B / kLanes));
-        if (!I.bitsT || !I.prev || !I.dead || !I.cond || !I.logp || !I.amp || !I.part || !I.counter)
+        if (!I.bitsT || !I.prev || !I.dead || !I.cond || !I.logp || !I.amp || !I.part)
             return bail(bfail(11, "batch buffer allocation failed"));
-        if (hipMemset(I.counter, 0, sizeof(unsigned) * size_t(B / kLanes)) != hipSuccess)
-            return bail(bfail(17, "counter initialisation failed"));
         bt->hinst.push_back(I);
     }
     bt->dinst = dalloc<BInst>(bt, bt->hinst.size());
@@ -760,6 +949,7 @@ void gs_batch_destroy(gs_batch* bt) {
     free_grad(bt);
     for (void* p : bt->allocs) (void)hipFree(p);
     for (auto* p : bt->ditems) (void)hipFree(p);
+    (void)hipFree(bt->dcnt);
     for (auto& ev : bt->step_ev) (void)hipEventDestroy(ev);
     (void)hipEventDestroy(bt->ev0);
     (void)hipEventDestroy(bt->ev1);
@@ -781,6 +971,14 @@ int gs_batch_set_tile(gs_batch* bt, int i, int k, int lg) {
     drop_graph(bt);
     (void)hipSetDevice(bt->device);
     return build_items(bt);
+}
+
+// log2 cosets each lane gathers at once (0..3; default 2 for complex64, 1 for complex128).
+int gs_batch_set_unroll(gs_batch* bt, int lu) {
+    if (!bt || lu < 0 || lu > 3) return bfail(16, "unroll must be 0..3");
+    bt->lu = lu;
+    drop_graph(bt);
+    return 0;
 }
 
 int gs_batch_use_graph(gs_batch* bt, int on) {
@@ -852,10 +1050,7 @@ int gs_batch_grad(gs_batch* bt, const int8_t* const* bits, double* const* log2p,
                   int seg, double mem_budget, int* seg_used) {
     if (!bt || !bits || !log2p || !G) return bfail(16, "bad argument");
     if (hipSetDevice(bt->device) != hipSuccess) return bfail(17, "hipSetDevice failed");
-    if (seg <= 0) {
-        seg = bt->maxns;
-        while (seg > 1 && double(grad_bytes(bt, seg)) > mem_budget) seg = (seg + 1) / 2;
-    }
+    if (seg <= 0) seg = choose_segment(bt, mem_budget);
     seg = std::min(std::max(seg, 1), bt->maxns);
     if (seg != bt->seg)
         if (int rc = alloc_grad(bt, seg)) return rc;

@@ -9,7 +9,7 @@ namespace gs {
 
 constexpr int kThreads = 256;       // threads per sweep block (4 waves of 64)
 constexpr int kMaxGridLog2 = 14;    // at most 16384 blocks per sweep; beyond that each block loops over tiles
-constexpr int kDefaultGridLog2 = 11;
+constexpr int kDefaultGridLog2 = 9;     // measured best for the width-30 sweeps (a few waves per SIMD suffice)
 constexpr int kFenceBlocks = 64;    // sweeps of at most this many blocks reduce in their last block
 constexpr int kPartStride = 8;      // doubles per block partial (write norms + fused next-step norms)
 constexpr size_t kPartDoubles = size_t(kPartStride) << kMaxGridLog2;

@@ -1096,7 +1096,7 @@ int gs_step_geometry(const gs_engine* eng, int k, int* out) {
     return 0;
 }
 
-// Cap the blocks of one sweep at 2^lg (1..14, default 11); tiles beyond the cap loop inside each block.  Rebuilds
+// Cap the blocks of one sweep at 2^lg (1..14, default 9); tiles beyond the cap loop inside each block.  Rebuilds
 // and uploads the step descriptors (synchronous).
 int gs_set_grid_log2(gs_engine* eng, int lg) {
     if (!eng) return fail(16, "null engine");

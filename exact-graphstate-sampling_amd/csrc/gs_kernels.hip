@@ -504,26 +504,37 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
 #pragma unroll
     for (int q = 0; q < NA; q++) acc[q] = 0;
 
-    uint64_t a_it = 0;
-    int p_it = 0;
-    for (int it = 0; it < niter; it++) {
-        if (it) {                                   // Gray code: one coordinate changes
-            const int b = __builtin_ctz(unsigned(it));
-            a_it ^= s.cols[jit + b];
-            p_it ^= pb(jit + b);
-        }
-        if (!active) continue;
-        const uint64_t ag = a_thr ^ a_blk ^ a_it;   // coset (ir, ig) of this thread has base ag ^ ir ^ ig * gam
-        const int pbase = p_thr ^ p_blk ^ p_it;
-        S e[NTQ][VG][VR];
+    // Software pipeline: the loads of tile it + 1 are issued before tile it is computed and stored.  Tiles hold
+    // disjoint cosets, and a coset's outputs go to its own slots (or to fresh slots outside the live space for growth
+    // steps), so reading the next tile early never reads a slot this thread or another writes in this step.
+    const uint64_t a_tb = a_thr ^ a_blk;
+    S e[NTQ][VG][VR];
+    auto load_tile = [&](uint64_t ag, S (&dst)[NTQ][VG][VR]) {
 #pragma unroll
         for (int j = 0; j < NTQ; j++)
 #pragma unroll
-            for (int ig = 0; ig < VG; ig++) {
-                const uint64_t a = ag ^ off[j] ^ (ig ? gam : 0);
-                Run<S, VR>::load(mem, a & ~uint64_t(VR - 1), e[j][ig]);
-                permute<S, VR>(e[j][ig], unsigned(a) & (VR - 1));
-            }
+            for (int ig = 0; ig < VG; ig++)
+                Run<S, VR>::load(mem, (ag ^ off[j] ^ (ig ? gam : 0)) & ~uint64_t(VR - 1), dst[j][ig]);
+    };
+    if (active) load_tile(a_tb, e);
+    uint64_t a_it = 0;
+    int p_it = 0;
+    for (int it = 0; it < niter; it++) {
+        const uint64_t ag = a_tb ^ a_it;            // coset (ir, ig) of this thread has base ag ^ ir ^ ig * gam
+        const int pbase = p_thr ^ p_blk ^ p_it;
+        S en[NTQ][VG][VR];
+        if (it + 1 < niter) {                       // Gray code: one coordinate changes
+            const int b = __builtin_ctz(unsigned(it + 1));
+            a_it ^= s.cols[jit + b];
+            p_it ^= pb(jit + b);
+            if (active) load_tile(a_tb ^ a_it, en);
+        }
+        if (!active) continue;
+#pragma unroll
+        for (int j = 0; j < NTQ; j++)
+#pragma unroll
+            for (int ig = 0; ig < VG; ig++)
+                permute<S, VR>(e[j][ig], unsigned(ag ^ off[j] ^ (ig ? gam : 0)) & (VR - 1));
         S o[2][VG][VR];
 #pragma unroll
         for (int ig = 0; ig < VG; ig++)
@@ -589,6 +600,12 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
                     else direct_acc<MO2>(A2, M, acc + 2);
                 }
         }
+#pragma unroll
+        for (int j = 0; j < NTQ; j++)
+#pragma unroll
+            for (int ig = 0; ig < VG; ig++)
+#pragma unroll
+                for (int ir = 0; ir < VR; ir++) e[j][ig][ir] = en[j][ig][ir];
     }
     sweep_epilogue<NA>(acc, part, ctl, epi, SWEEP);
 }

@@ -22,7 +22,7 @@ ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy
                "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
                "gs_last_device_ms", "gs_last_error", "gs_profile_enable", "gs_profile_read",
                "gs_step_geometry", "gs_set_fusion", "gs_set_grid_log2", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
-               "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_last_ms", "gs_batch_last_error",
+               "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms")
 
 _lib = None
@@ -80,6 +80,7 @@ def library():
         "gs_batch_score": (I, [P, ctypes.POINTER(i8p), ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp)]),
         "gs_batch_use_graph": (I, [P, I]),
         "gs_batch_set_tile": (I, [P, I, I, I]),
+        "gs_batch_set_unroll": (I, [P, I]),
         "gs_batch_last_ms": (D, [P, dp]),
         "gs_batch_last_error": (ctypes.c_char_p, []),
     }
@@ -263,7 +264,7 @@ class Engine:
         _check(library().gs_set_fusion(self._h, 1 if on else 0))
 
     def set_grid_log2(self, lg):
-        """Cap every sweep at 2^lg blocks (1..14, default 11); tiles beyond the cap loop inside each block."""
+        """Cap every sweep at 2^lg blocks (1..14, default 9); tiles beyond the cap loop inside each block."""
         _check(library().gs_set_grid_log2(self._h, int(lg)))
 
 
@@ -309,6 +310,10 @@ class BatchEngine:
     def set_tile(self, i, lg, k=-1):
         self._check(library().gs_batch_set_tile(self._h, i, k, lg))
 
+    def set_unroll(self, lu):
+        """log2 cosets each lane gathers at once (0..3)."""
+        self._check(library().gs_batch_set_unroll(self._h, int(lu)))
+
     def score(self, bits, want_cond=True, want_amp=True):
         """bits: list (per plan) of int8 [lanes_i, ns_i].  Returns lists of log2p [lanes_i], cond [lanes_i, ns_i],
         amp [lanes_i] complex."""
@@ -343,9 +348,10 @@ class BatchEngine:
         flat = np.ascontiguousarray(np.stack([M.real, M.imag], axis=-1).reshape(-1))
         self._check(library().gs_batch_set_matrices(self._h, i, _f8(flat)))
 
-    def grad(self, bits, seg=0, mem_budget=64e9):
+    def grad(self, bits, seg=0, mem_budget=0):
         """Adjoint: per plan ``(log2p [B], Z [B] complex, G [B, ns, 2] complex)`` with G[b, k, x] = dZ_b/dM_k[b_k, x];
-        checkpoint segments of ``seg`` steps (0: largest fitting ``mem_budget`` bytes).  Returns (results, seg)."""
+        checkpoint segments of ``seg`` steps (0: the segment that recomputes least within ``mem_budget`` bytes of
+        copies; budget 0: 70 % of the free device memory).  Returns (results, seg)."""
         n = len(self.plans)
         bs = [np.ascontiguousarray(np.asarray(b, dtype=np.int8)) for b in bits]
         lp = [np.empty(ln) for ln in self.lanes]

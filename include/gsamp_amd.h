@@ -95,13 +95,16 @@ int gs_batch_score(gs_batch* batch, const int8_t* const* bits, double* const* lo
 int gs_batch_use_graph(gs_batch* batch, int on);
 /* log2 cosets per work item of instance i at step k (k < 0: every step); not below the creation default. */
 int gs_batch_set_tile(gs_batch* batch, int i, int k, int lg);
+/* log2 cosets each lane gathers at once (0..3; default 2 for complex64, 1 for complex128). */
+int gs_batch_set_unroll(gs_batch* batch, int lu);
 /* Device time of the last gs_batch_score (ms); step_end_ms (may be NULL) [max ns]: end of each step after the start. */
 double gs_batch_last_ms(const gs_batch* batch, double* step_end_ms);
 /* Replace instance i's measurement matrices M[ns][2][2] (complex as re, im pairs; M[k][b][x]). */
 int gs_batch_set_matrices(gs_batch* batch, int i, const double* M);
 /* Adjoint (no counterpart in the reference): G[i] [lanes_i][ns_i][2] complex, G_k(x) = dZ/dM_k[b_k, x] for every
- * string, with checkpoint-and-recompute of the in-place table every seg steps (seg 0: the largest segment whose
- * copies fit in mem_budget bytes); also log2p[i] and amp[i] (may be NULL).  *seg_used receives the segment. */
+ * string, with checkpoint-and-recompute of the in-place table every seg steps (seg 0: the segment length that
+ * recomputes least while its copies fit in mem_budget bytes; mem_budget <= 0: 70 % of the free device memory); also
+ * log2p[i] and amp[i] (may be NULL).  *seg_used receives the segment. */
 int gs_batch_grad(gs_batch* batch, const int8_t* const* bits, double* const* log2p, double* const* amp,
                   double* const* G, int seg, double mem_budget, int* seg_used);
 /* Device time (ms) of the last gs_batch_grad; *forward_ms (may be NULL) its first forward pass. */
@@ -121,7 +124,7 @@ int gs_profile_read(const gs_engine* eng, double* ms, int64_t* launches);
 int gs_step_geometry(const gs_engine* eng, int k, int* out);
 /* Fold the next step's norm pass into the previous write (on, default) or run it as its own sweep (off). */
 int gs_set_fusion(gs_engine* eng, int on);
-/* Cap one sweep at 2^lg blocks (1..14, default 11); tiles beyond the cap loop inside each block. */
+/* Cap one sweep at 2^lg blocks (1..14, default 9); tiles beyond the cap loop inside each block. */
 int gs_set_grid_log2(gs_engine* eng, int lg);
 /* One plain device copy of `bytes` (read + write), *ms its duration: calibrates hardware byte counters. */
 int gs_copy_probe(int device, int64_t bytes, double* ms);
