@@ -236,7 +236,9 @@ def workload_c3(gs, egs, args):
         t1 = (time.perf_counter() - t0) / n1
     log(f"c3: one string per launch sequence: {1 / t1:.0f} amplitudes/s")
     with gs.BatchEngine([path], [B], storage=gs.C64) as bt:
+        t0 = time.perf_counter()
         tt = at.tune_tiles(bt, [strings[:B]], cache=cache)
+        log(f"c3: tiles tuned in {time.perf_counter() - t0:.1f} s (unroll {tt['lu']})")
         bt.use_graph(True)
         bt.score([strings[:B]], want_cond=False, want_amp=False)
         lps = []
@@ -244,10 +246,12 @@ def workload_c3(gs, egs, args):
         for i in range(0, N, B):
             lp, _, _ = bt.score([strings[i:i + B]], want_cond=False, want_amp=False)
             lps.append(lp[0])
+            log(f"c3 batched: {i + B} strings, {(i + B) / (time.perf_counter() - t0):.0f} amplitudes/s")
         tb = time.perf_counter() - t0
     lpb = np.concatenate(lps)
     res["single"] = {"amplitudes_per_s": 1 / t1, "strings_timed": n1}
-    res["batched"] = {"amplitudes_per_s": N / tb, "tiles": tt, "speedup_vs_single": (N / tb) * t1,
+    res["batched"] = {"amplitudes_per_s": N / tb, "tiles": {"lg": tt["lg"], "lu": tt["lu"]},
+                      "speedup_vs_single": (N / tb) * t1,
                       "max_abs_dlog2p_vs_single": float(np.max(np.abs(lpb[:n1] - lp1)))}
     log(f"c3 batched: {N / tb:.0f} amplitudes/s")
     # mixed instances
@@ -259,9 +263,11 @@ def workload_c3(gs, egs, args):
         plans.append(gs.Plan(path=pth))
         widths.append(mi.R)
         bits.append(np.random.default_rng(1000 + j).integers(0, 2, size=(64, len(mi.steps))).astype(np.int8))
-    best, sched, _ = at.tune_schedule(gs, plans, widths, 64, bits, args.c3_budget * 1e9, gs.C64, tile_cache=cache)
+    log(f"c3 mixed: {len(plans)} instances, widths {min(widths)}..{max(widths)}")
+    best, sched, _, tiles = at.tune_schedule(gs, plans, widths, 64, bits, args.c3_budget * 1e9, gs.C64,
+                                             tile_cache=cache, log=log)
     res["mixed"] = {"instances": len(plans), "lanes": 64, "widths": [min(widths), max(widths)],
-                    "budget_gb": args.c3_budget, "best_schedule": best, "schedules": sched}
+                    "budget_gb": args.c3_budget, "best_schedule": best, "schedules": sched, "unroll": tiles["lu"]}
     log(f"c3 mixed: {best} {sched[best]}")
     return res
 

@@ -133,19 +133,25 @@ def schedules(sizes, budget):
     return out
 
 
-def run_schedule(gs, plans, lanes, bits, groups, storage, tile_cache=None, graph=True):
-    """Score every instance once following ``groups`` (batches run one after another).  Returns dict(score_s: summed
-    scoring wall time, setup_s: engine creation and tile tuning, latency: per-instance seconds from the start of the
-    first batch's scoring to the end of the instance's last step, counting only scoring time), and log2p per
-    instance."""
+def apply_tiles(bt, tiles):
+    """Apply a tuned ``dict(lg=[per step index], lu=...)`` to every instance of a batch."""
+    _apply(bt, [tiles["lg"][k] if k < len(tiles["lg"]) else tiles["lg"][-1] for k in range(bt.maxns)])
+    bt.set_unroll(tiles["lu"])
+
+
+def run_schedule(gs, plans, lanes, bits, groups, storage, tiles, graph=True, log=None):
+    """Score every instance once following ``groups`` (batches run one after another) with the tiles ``tiles``.
+    Returns dict(score_s: summed scoring wall time, setup_s: engine creation and graph capture, latency:
+    per-instance seconds from the start of the first batch's scoring to the end of the instance's last step, counting
+    only scoring time), and log2p per instance."""
     lat = [0.0] * len(plans)
     lps = [None] * len(plans)
     score_s = setup_s = 0.0
-    for grp in groups:
+    for gi, grp in enumerate(groups):
         t0 = time.perf_counter()
         with gs.BatchEngine([plans[i] for i in grp], [lanes] * len(grp), storage=storage) as bt:
             sub = [bits[i] for i in grp]
-            tune_tiles(bt, sub, cache=tile_cache)
+            apply_tiles(bt, tiles)
             bt.use_graph(graph)
             bt.score(sub, want_cond=False, want_amp=False)          # warm (graph capture)
             setup_s += time.perf_counter() - t0
@@ -160,21 +166,32 @@ def run_schedule(gs, plans, lanes, bits, groups, storage, tile_cache=None, graph
                 lat[i] = score_s + min(1.0, frac) * dt
                 lps[i] = lp[j]
             score_s += dt
+        if log:
+            log(f"  batch {gi + 1}/{len(groups)}: {len(grp)} instances, {dt * 1e3:.1f} ms")
     return dict(score_s=score_s, setup_s=setup_s, latency=lat), lps
 
 
-def tune_schedule(gs, plans, widths, lanes, bits, budget, storage, tile_cache=None):
-    """Run every candidate schedule; returns (best name, {name: figures}, the best schedule's log2p per instance)."""
+def tune_schedule(gs, plans, widths, lanes, bits, budget, storage, tile_cache=None, log=None):
+    """Tune the tiles once on the largest batch of the first-fit schedule, then run every candidate schedule with
+    them; returns (best name, {name: figures}, the best schedule's log2p per instance, tiles)."""
     esz = 8 if storage == gs.C64 else 16
     sizes = [arena_bytes(w, lanes, esz) for w in widths]
+    cands = schedules(sizes, budget)
+    big = max(cands["first_fit_decreasing"], key=len)
+    with gs.BatchEngine([plans[i] for i in big], [lanes] * len(big), storage=storage) as bt:
+        tiles = tune_tiles(bt, [bits[i] for i in big], cache=tile_cache)
+    if log:
+        log(f"tiles tuned on a batch of {len(big)} instances: lu {tiles['lu']}")
     res = {}
     best = None
-    for name, groups in schedules(sizes, budget).items():
-        r, lps = run_schedule(gs, plans, lanes, bits, groups, storage, tile_cache)
+    for name, groups in cands.items():
+        if log:
+            log(f"schedule {name}: {len(groups)} batches")
+        r, lps = run_schedule(gs, plans, lanes, bits, groups, storage, tiles, log=log)
         lat = r["latency"]
         res[name] = dict(score_s=r["score_s"], setup_s=r["setup_s"], batches=len(groups),
                          amplitudes_per_s=len(plans) * lanes / r["score_s"],
                          p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)))
         if best is None or r["score_s"] < res[best]["score_s"]:
             best, best_lps = name, lps
-    return best, res, best_lps
+    return best, res, best_lps, tiles

@@ -487,11 +487,13 @@ struct gs_batch {
     std::vector<BItem*> ditems;
     std::vector<int> nitems;
     std::vector<std::vector<int>> lg;     // [inst][step]
-    int lg_default = 8;
+    int lg_default = 8;                   // smallest tile (sizes the partial buffers); steps start at 2^9
     std::vector<int> nq;                  // per step: most slots per coset among the instances (kernel variant)
     int* dcnt = nullptr;                  // device [maxns][ninst]: items per lane group of instance i at step k
     int maxgroups = 1;
     int lu = 2;                           // log2 cosets a lane gathers at once
+    bool items_dirty = false;             // tiles changed since the work lists were built
+    double auto_budget = -1;              // mem_budget argument of the last automatic segment choice
     // graph replay
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -571,12 +573,13 @@ int build_items(gs_batch* bt) {
     return 0;
 }
 
-// Largest number of items any (instance, group) may have in a step: sizes the partial buffers.
+// Largest number of items any (instance, group) may have in a step at the smallest tile gs_batch_set_tile accepts
+// (2^lg_default cosets): sizes the partial buffers.
 int max_items(const gs_batch* bt, size_t i) {
     int m = 1;
     for (int k = 0; k < bt->plans[i].ns; k++) {
         const int nk = bt->plans[i].steps[size_t(k)].nk;
-        m = std::max(m, 1 << (nk - std::min(nk, bt->lg[i][size_t(k)])));
+        m = std::max(m, 1 << (nk - std::min(nk, bt->lg_default)));
     }
     return m;
 }
@@ -876,7 +879,7 @@ int gs_batch_create(const gs_plan* const* plans, const int* lanes, int nplans, i
         bt->maxns = std::max(bt->maxns, plans[i]->p.ns);
     }
     bt->lg.resize(size_t(nplans));
-    for (int i = 0; i < nplans; i++) bt->lg[size_t(i)].assign(size_t(bt->plans[size_t(i)].ns), bt->lg_default);
+    for (int i = 0; i < nplans; i++) bt->lg[size_t(i)].assign(size_t(bt->plans[size_t(i)].ns), bt->lg_default + 1);
     if (hipStreamCreate(&bt->stream) != hipSuccess || hipEventCreate(&bt->ev0) != hipSuccess ||
         hipEventCreate(&bt->ev1) != hipSuccess)
         return bail(bfail(17, "stream creation failed"));
@@ -969,8 +972,8 @@ int gs_batch_set_tile(gs_batch* bt, int i, int k, int lg) {
         bt->lg[size_t(i)][size_t(kk)] = lg;
     }
     drop_graph(bt);
-    (void)hipSetDevice(bt->device);
-    return build_items(bt);
+    bt->items_dirty = true;                               // rebuilt once, before the next run
+    return 0;
 }
 
 // log2 cosets each lane gathers at once (0..3; default 2 for complex64, 1 for complex128).
@@ -1011,6 +1014,10 @@ int gs_batch_score(gs_batch* bt, const int8_t* const* bits, double* const* log2p
                    double* const* amp) {
     if (!bt || !bits || !log2p) return bfail(16, "bad argument");
     if (hipSetDevice(bt->device) != hipSuccess) return bfail(17, "hipSetDevice failed");
+    if (bt->items_dirty) {
+        if (int rc = build_items(bt)) return rc;
+        bt->items_dirty = false;
+    }
     if (int rc = upload_bits(bt, bits)) return rc;
     if (hipEventRecord(bt->ev0, bt->stream) != hipSuccess) return bfail(17, "event record failed");
     if (bt->use_graph) {
@@ -1050,7 +1057,17 @@ int gs_batch_grad(gs_batch* bt, const int8_t* const* bits, double* const* log2p,
                   int seg, double mem_budget, int* seg_used) {
     if (!bt || !bits || !log2p || !G) return bfail(16, "bad argument");
     if (hipSetDevice(bt->device) != hipSuccess) return bfail(17, "hipSetDevice failed");
-    if (seg <= 0) seg = choose_segment(bt, mem_budget);
+    if (bt->items_dirty) {
+        if (int rc = build_items(bt)) return rc;
+        bt->items_dirty = false;
+    }
+    if (seg <= 0) {                                       // automatic: chosen once per budget argument
+        if (bt->seg && bt->auto_budget == mem_budget) seg = bt->seg;
+        else seg = choose_segment(bt, mem_budget);
+        bt->auto_budget = mem_budget;
+    } else {
+        bt->auto_budget = -1;
+    }
     seg = std::min(std::max(seg, 1), bt->maxns);
     if (seg != bt->seg)
         if (int rc = alloc_grad(bt, seg)) return rc;
