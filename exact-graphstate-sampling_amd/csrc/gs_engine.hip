@@ -209,9 +209,11 @@ struct gs_engine {
     // per step: the plain descriptor (norm and write sweeps) and, where the next step's norms can be folded into
     // this step's write, the fused descriptor (coordinates relabelled so that each thread owns the pairs)
     std::vector<KStep> hplain, hfused;
+    std::vector<KStep> hlogical;    // plain steps in the plan's own coset coordinates (prefix/resume dump order)
     std::vector<char> fusable;
     KStep* dplain = nullptr;
     KStep* dfused = nullptr;
+    KStep* dlogical = nullptr;
     struct Launch {
         void* gram = nullptr;
         void* direct = nullptr;
@@ -427,6 +429,64 @@ bool gf2_solve(const std::vector<uint64_t>& basis, uint64_t x, uint64_t& coef) {
 
 int parity64(uint64_t x) { return __builtin_parityll(x); }
 
+// GF(2) echelon set of coordinate-space vectors (nk <= 40 bits): add() keeps a vector when it raises the rank.
+struct Echelon {
+    std::vector<uint64_t> rows;                     // reduced, distinct leading bits
+    bool add(uint64_t v) {
+        for (uint64_t r : rows)
+            if ((v >> (63 - __builtin_clzll(r))) & 1) v ^= r;
+        if (!v) return false;
+        rows.push_back(v);
+        std::sort(rows.begin(), rows.end(), [](uint64_t a, uint64_t b) { return a > b; });
+        return true;
+    }
+};
+
+// A new basis of a step's coset coordinates (vectors in the plan's coordinate space, bit j = coordinate j): `lead`
+// first, then the coordinates whose physical columns are the unit bits 2^from, 2^(from+1), ... for as long as each is
+// in the live span and independent, then the plan's own coordinates where they raise the rank.  The sweep is
+// invariant under such a relabelling c = P c' (columns T P, sign mask P^T rhoc): it only reorders the cosets, and
+// putting the low physical bits first makes each thread's cosets one aligned run.  Returns false when `lead` is
+// dependent.
+bool coord_basis(const std::vector<uint64_t>& cols, const std::vector<uint64_t>& lead, int from,
+                 std::vector<uint64_t>& basis) {
+    const int nk = int(cols.size());
+    Echelon ech;
+    basis.clear();
+    for (uint64_t v : lead) {
+        if (!ech.add(v)) return false;
+        basis.push_back(v);
+    }
+    for (int i = from; i < 64 && int(basis.size()) < nk; i++) {
+        uint64_t u = 0;
+        if (!gf2_solve(cols, uint64_t(1) << i, u) || !ech.add(u)) break;
+        basis.push_back(u);
+    }
+    for (int j = 0; j < nk && int(basis.size()) < nk; j++)
+        if (ech.add(uint64_t(1) << j)) basis.push_back(uint64_t(1) << j);
+    return int(basis.size()) == nk;
+}
+
+// Columns and sign mask of a step in the coordinates `basis`.
+void apply_basis(const std::vector<uint64_t>& cols, uint64_t rhoc, const std::vector<uint64_t>& basis, KStep& ks) {
+    memset(ks.cols, 0, sizeof ks.cols);
+    ks.pmask[0] = 0;
+    for (size_t i = 0; i < basis.size(); i++) {
+        uint64_t col = 0;
+        for (size_t j = 0; j < cols.size(); j++)
+            if ((basis[i] >> j) & 1) col ^= cols[j];
+        ks.cols[i] = col;
+        ks.pmask[0] |= uint64_t(parity64(basis[i] & rhoc)) << i;
+    }
+}
+
+// Number of leading columns equal to 1, 2, 4, ...
+int unit_lead(const KStep& ks) {
+    int u = 0;
+    while (u < ks.nk && ks.cols[u] == (uint64_t(1) << u)) u++;
+    return u;
+}
+
 void set_geometry(KStep& ks, int lv, int cap) {
     const int lt = lv + 8;
     if (ks.nk <= lt) {
@@ -449,8 +509,13 @@ bool build_fused(const gs_engine* e, int k, KStep& f) {
     const KStep& p1 = e->hplain[size_t(k + 1)];
     if (p1.ntq != 2 || p1.growth) return false;
     const int nk = s0.nk, lrf = fused_lr(e->storage);
-    for (int i = 0; i < lrf; i++)
-        if (i >= nk || s0.cols[size_t(i)] != (uint64_t(1) << i)) return false;
+    // coordinates of the first lrf unit physical bits (one aligned run per thread)
+    std::vector<uint64_t> lead;
+    for (int i = 0; i < lrf; i++) {
+        uint64_t u = 0;
+        if (i >= nk || !gf2_solve(s0.cols, uint64_t(1) << i, u)) return false;
+        lead.push_back(u);
+    }
     // live space after step k: kept columns, then g
     std::vector<uint64_t> live(s0.cols.begin(), s0.cols.end());
     if (s0.m == 2) live.push_back(s0.g);
@@ -460,14 +525,10 @@ bool build_fused(const gs_engine* e, int k, KStep& f) {
     const uint64_t cmask = nk >= 64 ? ~uint64_t(0) : (uint64_t(1) << nk) - 1;
     const uint64_t gamma = coef & cmask;
     const int eps = s0.m == 2 ? int((coef >> nk) & 1) : 0;
-    if ((gamma >> lrf) == 0) return false;
-    const int ptop = 63 - __builtin_clzll(gamma);
-    // coordinates: unit bits 0..lrf-1, gamma, then the remaining unit vectors except ptop
+    // coordinates: the unit-bit run, gamma (the pairing coordinate), then further unit bits and the rest
+    lead.push_back(gamma);
     std::vector<uint64_t> basis;
-    for (int i = 0; i < lrf; i++) basis.push_back(uint64_t(1) << i);
-    basis.push_back(gamma);
-    for (int j = lrf; j < nk; j++)
-        if (j != ptop) basis.push_back(uint64_t(1) << j);
+    if (!coord_basis(s0.cols, lead, lrf, basis)) return false;
     // slot index (phi) and sign (nu) of the next step as functionals of a physical address in the live space
     const StepInfo& s1 = P.steps[size_t(k + 1)];
     std::vector<uint64_t> nb(s1.cols.begin(), s1.cols.end());
@@ -518,6 +579,7 @@ bool build_fused(const gs_engine* e, int k, KStep& f) {
 int build_steps(gs_engine* e) {
     const Plan& P = e->plan;
     e->hplain.assign(size_t(P.ns), KStep{});
+    e->hlogical.assign(size_t(P.ns), KStep{});
     e->hfused.assign(size_t(P.ns), KStep{});
     e->fusable.assign(size_t(P.ns), 0);
     e->launch.assign(size_t(P.ns), gs_engine::Launch{});
@@ -541,10 +603,13 @@ int build_steps(gs_engine* e) {
         ks.vg = 1;
         std::string err;
         if (int rc = route_and_relabel(si, ks, err)) return fail(rc, err + " at step " + std::to_string(k));
-        // a run of 2^lrmax cosets per thread when the low kept columns are the unit bits 0..lrmax-1
-        int unit = 0;
-        while (unit < si.nk && unit < lrmax && si.cols[size_t(unit)] == (uint64_t(1) << unit)) unit++;
-        ks.lr = unit == lrmax ? lrmax : 0;
+        e->hlogical[size_t(k)] = ks;                     // plan coordinates (dump order)
+        // coordinates with the unit physical bits first: a run of 2^lrmax cosets per thread when 1 .. 2^(lrmax-1)
+        // are in the live span
+        std::vector<uint64_t> basis;
+        coord_basis(si.cols, {}, 0, basis);
+        apply_basis(si.cols, si.rhoc, basis, ks);
+        ks.lr = unit_lead(ks) >= lrmax ? lrmax : 0;
         set_geometry(ks, ks.lr, e->grid_cap);
         auto& L = e->launch[size_t(k)];
         L.grid = 1 << ks.lg_grid;
@@ -574,6 +639,7 @@ void free_engine(gs_engine* e) {
     (void)hipFree(e->arena);
     (void)hipFree(e->dplain);
     (void)hipFree(e->dfused);
+    (void)hipFree(e->dlogical);
     (void)hipFree(e->part);
     (void)hipFree(e->counter);
     (void)hipFree(e->ctl);
@@ -745,6 +811,7 @@ int gs_engine_create(const gs_plan* plan, int storage, int device, gs_engine** o
     const size_t C = gs_engine::kChunk;
     const size_t nsz = size_t(std::max(ns, 1));
     bool ok = dev_alloc((void**)&e->dplain, sizeof(KStep) * nsz) && dev_alloc((void**)&e->dfused, sizeof(KStep) * nsz) &&
+              dev_alloc((void**)&e->dlogical, sizeof(KStep) * nsz) &&
               dev_alloc((void**)&e->part, sizeof(double) * kPartDoubles) &&
               dev_alloc((void**)&e->counter, sizeof(unsigned) * kCounters) &&
               dev_alloc((void**)&e->ctl, sizeof(Ctl)) && dev_alloc((void**)&e->d_forced, C * nsz) &&
@@ -759,7 +826,8 @@ int gs_engine_create(const gs_plan* plan, int storage, int device, gs_engine** o
         hipEventCreate(&e->ev1) != hipSuccess)
         return bail(fail(17, "stream/event creation failed"));
     if (ns > 0 && (hipMemcpy(e->dplain, e->hplain.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess ||
-                   hipMemcpy(e->dfused, e->hfused.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess))
+                   hipMemcpy(e->dfused, e->hfused.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess ||
+                   hipMemcpy(e->dlogical, e->hlogical.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess))
         return bail(fail(17, "plan upload failed"));
     if (hipMemset(e->counter, 0, sizeof(unsigned) * kCounters) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
         return bail(fail(17, "counter initialisation failed"));
@@ -893,7 +961,7 @@ int gs_prefix(gs_engine* eng, const int8_t* bits, int K, double* table_reim, int
         return fail(11, "allocation of the dump buffer failed");
     }
     HIP_TRY(hipMemcpy(doffs, offs.data(), sizeof(uint64_t) * offs.size(), hipMemcpyHostToDevice));
-    const KStep* sk = K < ns ? e->dplain + K : nullptr;
+    const KStep* sk = K < ns ? e->dlogical + K : nullptr;
     const int grid = int(std::min<int64_t>((n + kThreads - 1) / kThreads, 65536));
     if (e->storage == GS_C64)
         hipLaunchKernelGGL(gather_kernel<float2>, dim3(grid), dim3(kThreads), 0, e->stream, sk,
@@ -977,7 +1045,7 @@ int gs_resume(gs_engine* eng, const int8_t* bits, int K, int nslices, const doub
                                cshift ? cshift[s] : 0ull, static_cast<const int*>(dperm) + size_t(s) * ntq, ntq, n);
         }
         if (rc) break;
-        const KStep* sk = K < ns ? e->dplain + K : nullptr;
+        const KStep* sk = K < ns ? e->dlogical + K : nullptr;
         if (e->storage == GS_C64)
             hipLaunchKernelGGL(scatter_kernel<float2>, dim3(grid), dim3(kThreads), 0, e->stream, sk,
                                static_cast<float2*>(e->arena), static_cast<const uint64_t*>(doffs), ntq, n,
@@ -1109,6 +1177,7 @@ int gs_set_grid_log2(gs_engine* eng, int lg) {
     if (ns) {
         HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(eng->dlogical, eng->hlogical.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
     }
     return 0;
 }

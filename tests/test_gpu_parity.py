@@ -271,3 +271,49 @@ def test_sliced_prefix_resume_with_twist(tmp_path, storage):
             _, lpf, _, _ = e.resume(b, 18, tabs, lss, 5, 0, cs, ts)
         got.append(lpf)
     assert close(got, want, 0) < TOL[storage] * 10
+
+
+@pytest.fixture(scope="module")
+def gadget_plan(tmp_path_factory):
+    """n = 48 data vertices, band 18, 24 phase-gadget ancillas (width 18): the kept columns of 43 steps are GF(2)
+    combinations whose span holds the low physical bits without being them, so the engine relabels those steps'
+    coset coordinates."""
+    d = tmp_path_factory.mktemp("g48")
+    inst = egs.instances.Instance(48, 18, 0, gadgets_count=24)
+    path = str(d / "plan.txt")
+    inst.write_plan(path)
+    return path, inst
+
+
+def test_relabelled_steps_vs_oracle(gadget_plan, tmp_path):
+    """Hybrid samples (ancillas forced 0, data sampled), prob of random strings and a prefix dump, FP64 arena, against
+    the oracle: bits identical, log2 Pr within 1e-12, the step-K table in the plan's own coset order within 1e-12."""
+    import subprocess
+    path, inst = gadget_plan
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    oracle = os.path.join(root, "oracle", "gs_oracle")
+    ns, na = inst.N, inst.N - inst.n
+    hy = np.full((4, ns), -1, np.int8)
+    hy[:, :na] = 0
+    rnd = np.random.default_rng(3).integers(0, 2, size=(4, ns)).astype(np.int8)
+    rnd[:, :na] = 0
+    egs.planio.write_strings(str(tmp_path / "hy.txt"), hy)
+    egs.planio.write_strings(str(tmp_path / "rnd.txt"), rnd)
+    run = lambda *a: [l.split() for l in subprocess.run([oracle, "--c128", path, *map(str, a)], capture_output=True,
+                                                        text=True, check=True).stdout.splitlines() if l.strip()]
+    ref_h = run("hybrid", str(tmp_path / "hy.txt"), 11)
+    ref_p = [float(r[0]) for r in run("prob", str(tmp_path / "rnd.txt"))]
+    K = na + 20
+    run("prefix", str(tmp_path / "rnd.txt"), K, str(tmp_path / "d"))
+    print("gadget n=48 band 18 + 24 ancillas, relabelled steps", flush=True)
+    with gs.Engine(path, gs.C128) as e:
+        assert sum(e.geometry(k)["lr"] > 0 for k in range(ns)) > 40
+        ob, lph, _ = e.hybrid(hy, 11)
+        lpp = e.prob(rnd)
+        tab, nk, ntq, ls, _ = e.prefix(rnd[0], K)
+    assert [egs.planio.format_bits(b) for b in ob] == [r[0] for r in ref_h]
+    assert close(lph, [float(r[1]) for r in ref_h], 0) < 1e-12
+    assert close(lpp, ref_p, 0) < 1e-12
+    dump, dnk, dntq, dls = egs.planio.read_dump(str(tmp_path / "d_0.bin"), c128=True)
+    assert (dnk, dntq) == (nk, ntq) and abs(dls - ls) < 1e-12
+    assert np.max(np.abs(np.asarray(dump) - tab)) < 1e-12 * max(1.0, np.max(np.abs(tab)))
