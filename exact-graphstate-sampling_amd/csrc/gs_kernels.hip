@@ -503,6 +503,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     double acc[NA];
 #pragma unroll
     for (int q = 0; q < NA; q++) acc[q] = 0;
+    const bool want_nb = epi.mode != kChain;
 
     // Software pipeline: the loads of tile it + 1 are issued before tile it is computed and stored.  Tiles hold
     // disjoint cosets, and a coset's outputs go to its own slots (or to fresh slots outside the live space for growth
@@ -557,8 +558,11 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
                         const double2 v = cfma(w0, A[t][0], cmul(w1, A[t][1]));
                         const S sv = Cx<S>::narrow(v);
                         o[t][ig][ir] = sv;
-                        acc[0] += norm2(v);
-                        acc[1] += norm2(Cx<S>::widen(sv));
+                        // the written norm feeds chain1 / run_steps bookkeeping only (sampling renormalises by the
+                        // decided branch's norm); the stored table's norm is read only by a growth step's analytic
+                        // norms, and a fused write is never followed by a growth step
+                        if (want_nb) acc[0] += norm2(v);
+                        if constexpr (!FUSED) acc[1] += norm2(Cx<S>::widen(sv));
                     }
                 }
             }
