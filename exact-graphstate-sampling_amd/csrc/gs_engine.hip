@@ -33,6 +33,8 @@ int fused_lr(int storage);
 void* decide_kernel_ptr();
 void* publish_kernel_ptr();
 void* reduce_kernel_ptr(int sweep);
+void* small_run_kernel_ptr(int storage);
+int small_run_threads();
 }  // namespace gs
 
 using namespace gs;
@@ -245,6 +247,7 @@ struct gs_engine {
     double last_ms = 0;
     bool allow_fusion = true;
     int grid_cap = kDefaultGridLog2;  // log2 of the most blocks a sweep launches
+    int small_nk = 10;              // runs of steps with nk <= small_nk go to one single-block launch (-1: never)
     // optional per-sweep timing (bench / profiling)
     bool profile = false;
     std::vector<hipEvent_t> pev;
@@ -366,6 +369,21 @@ int enqueue_run(gs_engine* e, int mode, int k0, int k1, const int8_t* forced_h, 
     auto fb = [&](int k) { return forced_h ? int(forced_h[k]) : -1; };
     bool decided = false;
     for (int k = k0; k < k1; k++) {
+        int ke = k;                                       // a run of two or more small steps: one launch
+        while (e->small_nk >= 0 && ke < k1 && e->hplain[size_t(ke)].nk <= e->small_nk) ke++;
+        if (ke - k >= 2) {
+            const KStep* sp = e->dplain;
+            void* mem = e->arena;
+            Ctl* ctl = e->ctl;
+            Epi ep = base;
+            int a = k, b = ke, d = decided ? 1 : 0;
+            void* args[] = {&sp, &mem, &ctl, &ep, &a, &b, &d};
+            HIP_TRY(hipLaunchKernel(small_run_kernel_ptr(e->storage), dim3(1), dim3(small_run_threads()), args, 0,
+                                    e->stream));
+            decided = false;
+            k = ke - 1;
+            continue;
+        }
         if (!decided) {
             const int nm = norm_for(e, mode, k, fb(k), fstr);
             if (nm >= 0) {
@@ -1179,6 +1197,14 @@ int gs_set_grid_log2(gs_engine* eng, int lg) {
         HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(eng->dlogical, eng->hlogical.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
     }
+    return 0;
+}
+
+// Steps with at most 2^nk cosets run, in runs of two or more, in one single-block launch (nk -1: never; default 10).
+int gs_set_small_steps(gs_engine* eng, int nk) {
+    if (!eng) return fail(16, "null engine");
+    if (nk < -1 || nk > 20) return fail(16, "small-step limit must be -1..20");
+    eng->small_nk = nk;
     return 0;
 }
 

@@ -315,13 +315,13 @@ __device__ void decide(Ctl& c, const Epi& e, int k, int norm, const double* red)
     c.w[3] = m1.y * sc;
 }
 
-// Block-wide sum of v[NQ] over the 256 threads in a fixed tree order; the result is in sh[0][*].
-template <int NQ>
-__device__ __forceinline__ void block_tree_sum(double (&sh)[kThreads][NQ], const double (&v)[NQ], int t) {
+// Block-wide sum of v[NQ] over the NT threads in a fixed tree order; the result is in sh[0][*].
+template <int NQ, int NT = kThreads>
+__device__ __forceinline__ void block_tree_sum(double (&sh)[NT][NQ], const double (&v)[NQ], int t) {
 #pragma unroll
     for (int q = 0; q < NQ; q++) sh[t][q] = v[q];
     __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
+    for (int w = NT / 2; w > 0; w >>= 1) {
         if (t < w)
 #pragma unroll
             for (int q = 0; q < NQ; q++) sh[t][q] += sh[t + w][q];
@@ -625,6 +625,116 @@ __global__ void decide_kernel(Ctl* ctl, Epi e, int k) {
 __global__ void publish_kernel(Ctl* ctl, Epi e) { publish(*ctl, e); }
 
 // ---------------------------------------------------------------------------------------------------------------
+// A run of small steps in one block.  The head and tail of a plan are steps of a few thousand cosets, where a sweep
+// costs far less than a launch; one block runs such a run step after step -- norm pass, decision, write pass,
+// bookkeeping -- with block barriers in place of launch boundaries.  The arithmetic per coset is the sweep kernels'
+// (FP64 routed slot sums, one rounding per store) and the chain functions are the same.
+
+constexpr int kSmallThreads = 1024;       // one block of 16 waves: enough loads in flight for 2^10-coset steps
+
+// One pass of a small step by the block's threads (cosets tid, tid + 1024, ...): sums into acc (Gram: 4, direct: 2,
+// write: written norm and stored norm).
+template <class S, int ROUTE, int NTQ>
+__device__ void small_pass(const KStep& s, S* mem, int sweep, const double2 (&M)[2][2], double2 w0, double2 w1,
+                           double (&acc)[4]) {
+    constexpr int MO = ROUTE == kRoute1 ? 1 : 2;
+    const uint64_t n = uint64_t(1) << s.nk;
+    for (uint64_t c = threadIdx.x; c < n; c += kSmallThreads) {
+        uint64_t base = 0;
+        for (int j = 0; j < s.nk; j++)
+            if ((c >> j) & 1) base ^= s.cols[j];
+        const int pc = __popcll(c & s.pmask[0]) & 1;
+        double2 wv[NTQ];
+#pragma unroll
+        for (int j = 0; j < NTQ; j++) wv[j] = Cx<S>::widen(mem[base ^ s.off[j]]);
+        double2 A[MO][2];
+        route_terms<ROUTE, NTQ, MO>(A, wv, pc, s.parmask);
+        if (sweep == kGram) {
+            gram_acc<MO>(A, acc);
+        } else if (sweep == kDirect) {
+            direct_acc<MO>(A, M, acc);
+        } else {
+#pragma unroll
+            for (int t = 0; t < MO; t++) {
+                const double2 v = cfma(w0, A[t][0], cmul(w1, A[t][1]));
+                const S sv = Cx<S>::narrow(v);
+                mem[t ? base ^ s.g : base] = sv;
+                acc[0] += norm2(v);
+                acc[1] += norm2(Cx<S>::widen(sv));
+            }
+        }
+    }
+}
+
+template <class S>
+__device__ void small_pass_any(const KStep& s, S* mem, int sweep, const double2 (&M)[2][2], double2 w0, double2 w1,
+                               double (&acc)[4]) {
+    switch (s.ntq * 4 + s.route) {
+        case 1 * 4 + kRoute1: small_pass<S, kRoute1, 1>(s, mem, sweep, M, w0, w1, acc); break;
+        case 1 * 4 + kRouteSplit: small_pass<S, kRouteSplit, 1>(s, mem, sweep, M, w0, w1, acc); break;
+        case 2 * 4 + kRoute1: small_pass<S, kRoute1, 2>(s, mem, sweep, M, w0, w1, acc); break;
+        case 2 * 4 + kRouteSplit: small_pass<S, kRouteSplit, 2>(s, mem, sweep, M, w0, w1, acc); break;
+        case 2 * 4 + kRouteMixA: small_pass<S, kRouteMixA, 2>(s, mem, sweep, M, w0, w1, acc); break;
+        case 2 * 4 + kRouteMixB: small_pass<S, kRouteMixB, 2>(s, mem, sweep, M, w0, w1, acc); break;
+        case 4 * 4 + kRoute1: small_pass<S, kRoute1, 4>(s, mem, sweep, M, w0, w1, acc); break;
+        case 4 * 4 + kRouteSplit: small_pass<S, kRouteSplit, 4>(s, mem, sweep, M, w0, w1, acc); break;
+        case 4 * 4 + kRouteMixA: small_pass<S, kRouteMixA, 4>(s, mem, sweep, M, w0, w1, acc); break;
+        case 4 * 4 + kRouteMixB: small_pass<S, kRouteMixB, 4>(s, mem, sweep, M, w0, w1, acc); break;
+    }
+}
+
+// Steps k0 .. k1-1 of one chain (one block of kSmallThreads).  decided0: step k0's decision was already made (by the
+// previous write's fused norms).  The norm choice per step is the host's (norm_for in the engine): chain1 and forced
+// run steps need none, growth steps use the stored norm, forced chain strings use direct norms, sampled steps Gram.
+template <class S>
+__global__ __launch_bounds__(kSmallThreads) void small_run_kernel(const KStep* __restrict__ steps, S* __restrict__ mem,
+                                                                  Ctl* __restrict__ ctl, Epi e, int k0, int k1,
+                                                                  int decided0) {
+    __shared__ double sh[kSmallThreads][4];
+    __shared__ int s_dead;
+    const int t = threadIdx.x;
+    Ctl& c = *ctl;
+    if (t == 0) s_dead = c.dead;
+    __syncthreads();
+    const bool fstr = e.forced != nullptr && e.mode == kChain;
+    for (int k = k0; k < k1 && !s_dead; k++) {
+        const KStep& s = steps[k];
+        double2 M[2][2];
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int x = 0; x < 2; x++) M[b][x] = make_double2(s.mre[b][x], s.mim[b][x]);
+        const double2 zero = make_double2(0, 0);
+        if (k > k0 || !decided0) {
+            const int fb = e.forced ? int(e.forced[k]) : -1;
+            int nm = kGram;
+            if (e.mode == kChain1 || (e.mode == kRun && fb >= 0) || s.growth) nm = -1;
+            else if (fstr) nm = kDirect;
+            double acc[4] = {0, 0, 0, 0};
+            if (nm >= 0) small_pass_any<S>(s, mem, nm, M, zero, zero, acc);
+            block_tree_sum<4, kSmallThreads>(sh, acc, t);
+            if (t == 0) {
+                const double red[4] = {sh[0][0], sh[0][1], sh[0][2], sh[0][3]};
+                decide(c, e, k, nm, nm >= 0 ? red : nullptr);
+                s_dead = c.dead;
+            }
+            __syncthreads();
+            if (s_dead) break;
+        }
+        const double2 w0 = make_double2(c.w[0], c.w[1]), w1 = make_double2(c.w[2], c.w[3]);
+        double acc[4] = {0, 0, 0, 0};
+        small_pass_any<S>(s, mem, kWrite, M, w0, w1, acc);
+        block_tree_sum<4, kSmallThreads>(sh, acc, t);
+        if (t == 0) {
+            finish_write(c, e, k, sh[0][0], sh[0][1]);
+            if (!c.dead && k + 1 == e.k1) publish(c, e);
+            s_dead = c.dead;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // dispatch table
 
 template <class S, int VR, int VG, int NTQ, int ROUTE, int SWEEP, int ROUTE2>
@@ -722,5 +832,10 @@ void* reduce_kernel_ptr(int sweep) {
     return nullptr;
 }
 void* publish_kernel_ptr() { return reinterpret_cast<void*>(&publish_kernel); }
+int small_run_threads() { return kSmallThreads; }
+void* small_run_kernel_ptr(int storage) {
+    return storage == 0 ? reinterpret_cast<void*>(&small_run_kernel<float2>)
+                        : reinterpret_cast<void*>(&small_run_kernel<double2>);
+}
 
 }  // namespace gs

@@ -21,7 +21,7 @@ SEED_PLAIN, SEED_SEQ = 0, 1
 ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy", "gs_engine_create",
                "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
                "gs_last_device_ms", "gs_last_error", "gs_profile_enable", "gs_profile_read",
-               "gs_step_geometry", "gs_set_fusion", "gs_set_grid_log2", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
+               "gs_step_geometry", "gs_set_fusion", "gs_set_grid_log2", "gs_set_small_steps", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
                "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms")
 
@@ -70,6 +70,7 @@ def library():
         "gs_step_geometry": (I, [P, I, ip]),
         "gs_set_fusion": (I, [P, I]),
         "gs_set_grid_log2": (I, [P, I]),
+        "gs_set_small_steps": (I, [P, I]),
         "gs_copy_probe": (I, [I, I64, dp]),
         "gs_batch_create": (I, [ctypes.POINTER(P), ip, I, I, I, pp]),
         "gs_batch_set_matrices": (I, [P, I, dp]),
@@ -262,6 +263,10 @@ class Engine:
     def set_fusion(self, on):
         """Fold each sampled step's norm pass into the previous write sweep where possible (default on)."""
         _check(library().gs_set_fusion(self._h, 1 if on else 0))
+
+    def set_small_steps(self, nk):
+        """Run consecutive steps of at most 2^nk cosets in one single-block launch (-1: never)."""
+        _check(library().gs_set_small_steps(self._h, int(nk)))
 
     def set_grid_log2(self, lg):
         """Cap every sweep at 2^lg blocks (1..14, default 9); tiles beyond the cap loop inside each block."""

@@ -126,6 +126,8 @@ int gs_step_geometry(const gs_engine* eng, int k, int* out);
 int gs_set_fusion(gs_engine* eng, int on);
 /* Cap one sweep at 2^lg blocks (1..14, default 9); tiles beyond the cap loop inside each block. */
 int gs_set_grid_log2(gs_engine* eng, int lg);
+/* Runs of two or more steps with at most 2^nk cosets go to one single-block launch (nk -1: never; default 10). */
+int gs_set_small_steps(gs_engine* eng, int nk);
 /* One plain device copy of `bytes` (read + write), *ms its duration: calibrates hardware byte counters. */
 int gs_copy_probe(int device, int64_t bytes, double* ms);
 

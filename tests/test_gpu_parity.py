@@ -33,6 +33,16 @@ def close(a, b, tol):
     return float(np.max(d)) if d.size else 0.0
 
 
+PATHS = ["small-runs", "fused", "plain"]
+
+
+def configure(e, path):
+    """small-runs: the default engine (runs of small steps in one block, folded norms elsewhere); fused: every step
+    its own sweeps, norms folded into the previous write; plain: one norm sweep per sampled step."""
+    e.set_small_steps(10 if path == "small-runs" else -1)
+    e.set_fusion(path != "plain")
+
+
 @pytest.fixture(scope="module")
 def c1(golden):
     d = os.path.join(golden, "c1")
@@ -44,13 +54,13 @@ def c1(golden):
     return d, meta, strings, hy
 
 
-@pytest.mark.parametrize("fusion", [True, False])
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("storage", [gs.C64, gs.C128])
-def test_c1_prob_chain(c1, storage, fusion):
+def test_c1_prob_chain(c1, storage, path):
     d, meta, strings, _ = c1
-    print(f"c1 prob/chain storage={TAG[storage]} fusion={fusion}", flush=True)
+    print(f"c1 prob/chain storage={TAG[storage]} path={path}", flush=True)
     with gs.Engine(os.path.join(d, "plan.txt"), storage) as e:
-        e.set_fusion(fusion)
+        configure(e, path)
         lp = e.prob(strings)
         ref = np.array([float(l[0]) for l in lines(os.path.join(d, f"prob_{TAG[storage]}.txt"))])
         assert close(lp, ref, 0) <= TOL[storage], close(lp, ref, 0)
@@ -62,17 +72,18 @@ def test_c1_prob_chain(c1, storage, fusion):
         assert close(np.log2(np.abs(amp) ** 2) - meta["ns"], lpc, 0) <= 10 * TOL[storage]
 
 
-@pytest.mark.parametrize("fusion", [True, False])
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("storage", [gs.C64, gs.C128])
 @pytest.mark.parametrize("mode", ["sample", "sample2"])
-def test_c1_sample_bits_identical(c1, storage, mode, fusion):
-    """Sampled bits identical to the reference; with the next step's norms folded into the previous write (default)
-    and with a separate norm sweep per sampled step."""
+def test_c1_sample_bits_identical(c1, storage, mode, path):
+    """Sampled bits identical to the reference on every execution path: the whole chain in one block (the plan's
+    steps are all small), separate sweeps with the next step's norms folded into the previous write, and one norm
+    sweep per sampled step."""
     d, meta, _, _ = c1
-    print(f"c1 {mode} storage={TAG[storage]} fusion={fusion}", flush=True)
+    print(f"c1 {mode} storage={TAG[storage]} path={path}", flush=True)
     with gs.Engine(os.path.join(d, "plan.txt"), storage) as e:
-        e.set_fusion(fusion)
-        if fusion:
+        configure(e, path)
+        if path == "fused":
             assert any(e.geometry(k)["fusable"] for k in range(e.ns))
         bits, lp = e.sample(16, meta["sample_seed"], seeding=gs.SEED_PLAIN if mode == "sample" else gs.SEED_SEQ)
     ref = lines(os.path.join(d, f"{mode}_{TAG[storage]}.txt"))
@@ -80,11 +91,13 @@ def test_c1_sample_bits_identical(c1, storage, mode, fusion):
     assert close(lp, [float(l[1]) for l in ref], 0) <= TOL[storage]
 
 
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("storage", [gs.C64, gs.C128])
-def test_c1_hybrid(c1, storage):
+def test_c1_hybrid(c1, storage, path):
     d, meta, _, hy = c1
-    print(f"c1 hybrid storage={TAG[storage]}", flush=True)
+    print(f"c1 hybrid storage={TAG[storage]} path={path}", flush=True)
     with gs.Engine(os.path.join(d, "plan.txt"), storage) as e:
+        configure(e, path)
         bits, lp, cond = e.hybrid(hy, meta["hybrid_seed"])
     ref = lines(os.path.join(d, f"hybrid_{TAG[storage]}.txt"))
     assert [egs.planio.format_bits(b) for b in bits] == [l[0] for l in ref]
@@ -93,14 +106,16 @@ def test_c1_hybrid(c1, storage):
 
 
 @pytest.mark.parametrize("storage", [gs.C64, gs.C128])
-def test_c1_prefix_resume(c1, storage):
+@pytest.mark.parametrize("path", ["small-runs", "fused"])
+def test_c1_prefix_resume(c1, storage, path):
     d, meta, _, hy = c1
     K = meta["K"]
     c128 = storage == gs.C128
-    print(f"c1 prefix/resume storage={TAG[storage]}", flush=True)
+    print(f"c1 prefix/resume storage={TAG[storage]} path={path}", flush=True)
     ref_pre = lines(os.path.join(d, f"prefix_{TAG[storage]}.txt"))
     ref_res = lines(os.path.join(d, f"resume_{TAG[storage]}.txt"))
     with gs.Engine(os.path.join(d, "plan.txt"), storage) as e:
+        configure(e, path)
         for li, row in enumerate(hy):
             tab, nk, ntq, ls, lp = e.prefix(row, K)
             rtab, rnk, rntq, rls = egs.planio.read_dump(os.path.join(d, f"dump_{TAG[storage]}_{li}.bin"), c128)
