@@ -13,6 +13,7 @@ BIN = os.path.join(HERE, "bin")
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(BIN, "libgsamp_amd.so")
 CLI = os.path.join(BIN, "gsamp2_amd")
+TT_CLIS = (os.path.join(BIN, "gsamp_amd"), os.path.join(BIN, "dp_amd"))
 ARCH = os.environ.get("GS_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -30,7 +31,7 @@ def _run(cmd, quiet_warn=True):
 def build_library():
     os.makedirs(BIN, exist_ok=True)
     objs = []
-    for src in ("gs_kernels.hip", "gs_engine.hip", "gs_batch.hip", "gs_plan.cpp"):
+    for src in ("gs_kernels.hip", "gs_engine.hip", "gs_batch.hip", "gs_twotable.hip", "gs_plan.cpp"):
         obj = os.path.join(BIN, src.split(".")[0] + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
         _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
@@ -47,6 +48,11 @@ def build_cli():
     _run(["g++", "-O2", "-std=c++17", "-o", tmp, os.path.join(CSRC, "gs_cli.cpp"),
           f"-L{BIN}", "-lgsamp_amd", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, CLI)
+    for out in TT_CLIS:                      # one source, the file name selects the gsamp or dp contract
+        tmp = out + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", "-o", tmp, os.path.join(CSRC, "gs_tt_cli.cpp"),
+              f"-L{BIN}", "-lgsamp_amd", "-Wl,-rpath,$ORIGIN"])
+        os.replace(tmp, out)
 
 
 def build_oracle():

@@ -49,6 +49,23 @@ struct KStep {
     int par2;              // fused: next step's par of its second offset
     int eps;               // fused: the partner of output tau is output tau ^ eps of the paired coset
     int phig, nug;         // fused: slot-index and sign parity of g
+    int rev;               // walk the tiles of each block in reverse Gray order
+    int pad;
+};
+
+// One step of the two-table (gather-form) engine as its kernels see it: output cp of the new table gathers the inputs
+// u = pdep(cp, pivmask) ^ kspan[t] of the old one (c = u & lowmask, x = the bit r of u).  pdep is linear in cp:
+// output coordinate j goes to the physical bit dep[j].
+struct TStep {
+    uint64_t dep[40];      // 1 << (position of the j-th set bit of pivmask)
+    uint64_t kspan[4];     // the 2^nker combinations of the kernel vectors
+    uint64_t lowmask;      // (1 << r) - 1
+    uint64_t xbit;         // 1 << r
+    uint64_t rho;          // sign mask over c
+    int rn;                // log2 outputs
+    int nks;               // 2^nker
+    int lg_grid;           // log2 blocks
+    int lg_iter;           // log2 tiles per block
 };
 
 // Chain state carried on the device between steps (one per engine).

@@ -248,6 +248,7 @@ struct gs_engine {
     bool allow_fusion = true;
     int grid_cap = kDefaultGridLog2;  // log2 of the most blocks a sweep launches
     int small_nk = 10;              // runs of steps with nk <= small_nk go to one single-block launch (-1: never)
+    int walk = 0;                   // tile order: 0 forward, 1 reverse on odd steps, 2 reverse on every step
     // optional per-sweep timing (bench / profiling)
     bool profile = false;
     std::vector<hipEvent_t> pev;
@@ -636,6 +637,7 @@ int build_steps(gs_engine* e) {
         L.write = sweep_kernel_for(e->storage, ks.lr, 1, ks.ntq, ks.route, kWrite, 0);
         if (!L.gram || !L.direct || !L.write) return fail(15, "no kernel for step " + std::to_string(k));
     }
+    for (int k = 0; k < P.ns; k++) e->hplain[size_t(k)].rev = e->walk == 2 || (e->walk == 1 && (k & 1));
     for (int k = 0; k + 1 < P.ns; k++) {
         KStep f;
         if (!build_fused(e, k, f)) continue;
@@ -1196,6 +1198,32 @@ int gs_set_grid_log2(gs_engine* eng, int lg) {
         HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(eng->dlogical, eng->hlogical.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+// Coset-coordinate columns of step k as the sweeps use them (fused: the fused layout), into cols[0 .. cap-1];
+// returns nk, or -1 (no such step / layout).
+int gs_step_columns(const gs_engine* eng, int k, int fused, uint64_t* cols, int cap) {
+    if (!eng || k < 0 || k >= eng->plan.ns || !cols) return -1;
+    if (fused && !eng->fusable[size_t(k)]) return -1;
+    const KStep& s = fused ? eng->hfused[size_t(k)] : eng->hplain[size_t(k)];
+    for (int i = 0; i < s.nk && i < cap; i++) cols[i] = s.cols[i];
+    return s.nk;
+}
+
+// Tile order of the sweeps: 0 forward, 1 reverse on odd steps, 2 reverse on every step (results do not depend on it).
+int gs_set_walk(gs_engine* eng, int mode) {
+    if (!eng) return fail(16, "null engine");
+    if (mode < 0 || mode > 2) return fail(16, "walk must be 0, 1 or 2");
+    HIP_TRY(hipSetDevice(eng->device));
+    HIP_TRY(hipStreamSynchronize(eng->stream));
+    eng->walk = mode;
+    if (int rc = build_steps(eng)) return rc;
+    const size_t ns = size_t(eng->plan.ns);
+    if (ns) {
+        HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
     }
     return 0;
 }

@@ -517,15 +517,22 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
             for (int ig = 0; ig < VG; ig++)
                 Run<S, VR>::load(mem, (ag ^ off[j] ^ (ig ? gam : 0)) & ~uint64_t(VR - 1), dst[j][ig]);
     };
-    if (active) load_tile(a_tb, e);
+    // Tile index i runs 0 .. niter-1, or niter-1 .. 0 when s.rev (gray(i) ^ gray(i-1) = 1 << ctz(i)): the start of a
+    // sweep then meets the end of the previous one, whose last lines may still be in the memory-side cache.
+    const bool rev = s.rev != 0;
     uint64_t a_it = 0;
     int p_it = 0;
+    if (rev && s.lg_iter > 0) {                     // gray(niter - 1) is the top bit alone
+        a_it = s.cols[jit + s.lg_iter - 1];
+        p_it = pb(jit + s.lg_iter - 1);
+    }
+    if (active) load_tile(a_tb ^ a_it, e);
     for (int it = 0; it < niter; it++) {
         const uint64_t ag = a_tb ^ a_it;            // coset (ir, ig) of this thread has base ag ^ ir ^ ig * gam
         const int pbase = p_thr ^ p_blk ^ p_it;
         S en[NTQ][VG][VR];
         if (it + 1 < niter) {                       // Gray code: one coordinate changes
-            const int b = __builtin_ctz(unsigned(it + 1));
+            const int b = __builtin_ctz(unsigned(rev ? niter - 1 - it : it + 1));
             a_it ^= s.cols[jit + b];
             p_it ^= pb(jit + b);
             if (active) load_tile(a_tb ^ a_it, en);
@@ -735,6 +742,87 @@ __global__ __launch_bounds__(kSmallThreads) void small_run_kernel(const KStep* _
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// Two-table (gather-form) steps (gsamp.cpp:21-29, :39-55; dp.cpp:44-53).  Output cp gathers its 2^nker inputs; S0 sums
+// the x = 0 terms, S1 the x = 1 terms with sign (-1)^{parity(c & rho)}; the outputs are M[b][0] S0 + M[b][1] S1.
+// Norm pass (SWEEP = kDirect): both branch norms |M[b][0] S0 + M[b][1] S1|^2 summed directly (the reference's chain).
+// Write pass (kWrite): w0 S0 + w1 S1 stored at nw[cp] (contiguous), with w from the chain's decision or, for the
+// amplitude engine, from wext[4k..4k+3] (the job's per-vertex weights).  Threads own consecutive outputs: output
+// bits 0..7 are the thread index, then the block index, then the Gray-ordered tile index.
+template <class S, int SWEEP>
+__global__ __launch_bounds__(kThreads) void gather_kernel(const TStep* __restrict__ tp, const S* __restrict__ old,
+                                                          S* __restrict__ nw, double* __restrict__ part,
+                                                          Ctl* __restrict__ ctl, Epi epi,
+                                                          const double* __restrict__ wext) {
+    if (ctl->dead) return;
+    const TStep& s = *tp;
+    const int tid = threadIdx.x;
+    uint64_t a_thr = 0;
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+        if (b < s.rn && ((tid >> b) & 1)) a_thr ^= s.dep[b];
+    const bool active = s.rn >= 8 || tid < (1 << s.rn);
+    uint64_t a_blk = 0;
+    for (int b = 0; b < s.lg_grid; b++)
+        if ((blockIdx.x >> b) & 1) a_blk ^= s.dep[8 + b];
+    const int jit = 8 + s.lg_grid;
+    const int niter = 1 << s.lg_iter;
+    double2 M[2][2], w0 = make_double2(0, 0), w1 = w0;
+    if constexpr (SWEEP == kDirect) {
+        const KStep& ms = epi.plain[epi.k];
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int x = 0; x < 2; x++) M[b][x] = make_double2(ms.mre[b][x], ms.mim[b][x]);
+    } else {
+        const double* w = wext ? wext + 4 * epi.k : ctl->w;
+        w0 = make_double2(w[0], w[1]);
+        w1 = make_double2(w[2], w[3]);
+    }
+    const int nks = s.nks;
+    uint64_t ks[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) ks[t] = s.kspan[t];
+    const uint64_t lowmask = s.lowmask, xbit = s.xbit, rho = s.rho;
+    double acc[2] = {0, 0};
+    uint64_t a_it = 0, c_it = 0;
+    for (int it = 0; it < niter; it++) {
+        if (it) {
+            const int b = __builtin_ctz(unsigned(it));
+            a_it ^= s.dep[jit + b];
+            c_it ^= uint64_t(1) << (jit + b);
+        }
+        if (!active) continue;
+        const uint64_t u0 = a_thr ^ a_blk ^ a_it;
+        double2 S0 = make_double2(0, 0), S1 = S0;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            if (t >= nks) break;
+            const uint64_t u = u0 ^ ks[t];
+            const uint64_t c = u & lowmask;
+            const double2 v = Cx<S>::widen(old[c]);
+            if (u & xbit) S1 = cadd(S1, negate_if(v, __popcll(c & rho) & 1));
+            else S0 = cadd(S0, v);
+        }
+        if constexpr (SWEEP == kDirect) {
+            acc[0] += norm2(cfma(M[0][0], S0, cmul(M[0][1], S1)));
+            acc[1] += norm2(cfma(M[1][0], S0, cmul(M[1][1], S1)));
+        } else {
+            const double2 o = cfma(w0, S0, cmul(w1, S1));
+            const S so = Cx<S>::narrow(o);
+            const uint64_t cp = c_it | (uint64_t(blockIdx.x) << 8) | uint64_t(tid);
+            Run<S, 1>::store(nw, cp, {so});
+            acc[0] += norm2(o);
+            acc[1] += norm2(Cx<S>::widen(so));
+        }
+    }
+    sweep_epilogue<2>(acc, part, ctl, epi, SWEEP);
+}
+
+// Entry 0 of a table set to 1 (start of a chain).
+template <class S>
+__global__ void table_one_kernel(S* t) { t[0] = Cx<S>::narrow(make_double2(1.0, 0.0)); }
+
+// ---------------------------------------------------------------------------------------------------------------
 // dispatch table
 
 template <class S, int VR, int VG, int NTQ, int ROUTE, int SWEEP, int ROUTE2>
@@ -833,6 +921,17 @@ void* reduce_kernel_ptr(int sweep) {
 }
 void* publish_kernel_ptr() { return reinterpret_cast<void*>(&publish_kernel); }
 int small_run_threads() { return kSmallThreads; }
+// Two-table kernels: sweep kDirect (norm pass) or kWrite; storage 0 = complex64, 1 = complex128.
+void* gather_kernel_ptr(int storage, int sweep) {
+    if (storage == 0) return sweep == kDirect ? reinterpret_cast<void*>(&gather_kernel<float2, kDirect>)
+                                              : reinterpret_cast<void*>(&gather_kernel<float2, kWrite>);
+    return sweep == kDirect ? reinterpret_cast<void*>(&gather_kernel<double2, kDirect>)
+                            : reinterpret_cast<void*>(&gather_kernel<double2, kWrite>);
+}
+void* table_one_kernel_ptr(int storage) {
+    return storage == 0 ? reinterpret_cast<void*>(&table_one_kernel<float2>)
+                        : reinterpret_cast<void*>(&table_one_kernel<double2>);
+}
 void* small_run_kernel_ptr(int storage) {
     return storage == 0 ? reinterpret_cast<void*>(&small_run_kernel<float2>)
                         : reinterpret_cast<void*>(&small_run_kernel<double2>);

@@ -21,9 +21,11 @@ SEED_PLAIN, SEED_SEQ = 0, 1
 ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy", "gs_engine_create",
                "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
                "gs_last_device_ms", "gs_last_error", "gs_profile_enable", "gs_profile_read",
-               "gs_step_geometry", "gs_set_fusion", "gs_set_grid_log2", "gs_set_small_steps", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
+               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_small_steps", "gs_set_walk", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
                "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_last_ms", "gs_batch_last_error",
-               "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms")
+               "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms", "gs_tt_plan_load", "gs_tt_plan_info",
+               "gs_tt_plan_destroy", "gs_tt_engine_create", "gs_tt_engine_destroy", "gs_tt_prob", "gs_tt_chain",
+               "gs_tt_sample", "gs_tt_hybrid", "gs_tt_dp", "gs_tt_last_error")
 
 _lib = None
 
@@ -69,8 +71,10 @@ def library():
         "gs_profile_read": (I, [P, dp, ctypes.POINTER(I64)]),
         "gs_step_geometry": (I, [P, I, ip]),
         "gs_set_fusion": (I, [P, I]),
+        "gs_step_columns": (I, [P, I, I, ctypes.POINTER(ctypes.c_uint64), I]),
         "gs_set_grid_log2": (I, [P, I]),
         "gs_set_small_steps": (I, [P, I]),
+        "gs_set_walk": (I, [P, I]),
         "gs_copy_probe": (I, [I, I64, dp]),
         "gs_batch_create": (I, [ctypes.POINTER(P), ip, I, I, I, pp]),
         "gs_batch_set_matrices": (I, [P, I, dp]),
@@ -82,6 +86,17 @@ def library():
         "gs_batch_use_graph": (I, [P, I]),
         "gs_batch_set_tile": (I, [P, I, I, I]),
         "gs_batch_set_unroll": (I, [P, I]),
+        "gs_tt_plan_load": (I, [ctypes.c_char_p, I, pp]),
+        "gs_tt_plan_info": (I, [P, ip, ip, ip]),
+        "gs_tt_plan_destroy": (None, [P]),
+        "gs_tt_engine_create": (I, [P, I, I, pp]),
+        "gs_tt_engine_destroy": (None, [P]),
+        "gs_tt_prob": (I, [P, i8p, I64, dp]),
+        "gs_tt_chain": (I, [P, i8p, I64, dp, dp, dp]),
+        "gs_tt_sample": (I, [P, U64, I64, I64, i8p, dp]),
+        "gs_tt_hybrid": (I, [P, i8p, I64, U64, I64, i8p, dp, dp]),
+        "gs_tt_dp": (I, [P, I64, ip, dp, dp, dp]),
+        "gs_tt_last_error": (ctypes.c_char_p, []),
         "gs_batch_last_ms": (D, [P, dp]),
         "gs_batch_last_error": (ctypes.c_char_p, []),
     }
@@ -260,9 +275,19 @@ class Engine:
         _check(library().gs_step_geometry(self._h, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         return dict(zip(("lr", "grid", "ntq", "route", "fusable", "fgrid", "route2", "lg_iter"), (int(x) for x in out)))
 
+    def columns(self, k, fused=False):
+        """Coset-coordinate columns of step k as the plain or fused sweep uses them (None: no such layout)."""
+        out = np.zeros(40, np.uint64)
+        nk = library().gs_step_columns(self._h, k, 1 if fused else 0, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 40)
+        return None if nk < 0 else [int(x) for x in out[:nk]]
+
     def set_fusion(self, on):
         """Fold each sampled step's norm pass into the previous write sweep where possible (default on)."""
         _check(library().gs_set_fusion(self._h, 1 if on else 0))
+
+    def set_walk(self, mode):
+        """Tile order of the sweeps: 0 forward, 1 reverse on odd steps, 2 reverse on every step."""
+        _check(library().gs_set_walk(self._h, int(mode)))
 
     def set_small_steps(self, nk):
         """Run consecutive steps of at most 2^nk cosets in one single-block launch (-1: never)."""
@@ -376,3 +401,93 @@ class BatchEngine:
         f = ctypes.c_double()
         t = library().gs_batch_grad_ms(self._h, ctypes.byref(f))
         return t, f.value
+
+
+class TwoTableEngine:
+    """The two-table (gather-form) engine: the reference's gsamp.cpp sampler (``with_m=True`` plans) and dp.cpp amplitude
+    engine (``with_m=False``); two tables of 2^maxr entries on one device."""
+
+    def __init__(self, path, with_m=True, storage=C64, device=0):
+        lib = library()
+        self._plan = ctypes.c_void_p()
+        self._h = None
+        rc = lib.gs_tt_plan_load(os.fsencode(path), 1 if with_m else 0, ctypes.byref(self._plan))
+        if rc:
+            raise GsError(rc, lib.gs_tt_last_error().decode())
+        n, ns, maxr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib.gs_tt_plan_info(self._plan, ctypes.byref(n), ctypes.byref(ns), ctypes.byref(maxr))
+        self.n, self.ns, self.maxr = n.value, ns.value, maxr.value
+        h = ctypes.c_void_p()
+        self._check(lib.gs_tt_engine_create(self._plan, storage, device, ctypes.byref(h)))
+        self._h = h
+
+    def _check(self, rc):
+        if rc:
+            raise GsError(rc, library().gs_tt_last_error().decode())
+
+    def close(self):
+        if _lib is not None:
+            if getattr(self, "_h", None):
+                _lib.gs_tt_engine_destroy(self._h)
+                self._h = None
+            if getattr(self, "_plan", None):
+                _lib.gs_tt_plan_destroy(self._plan)
+                self._plan = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _bits(self, bits, allow_q=False):
+        b = np.ascontiguousarray(np.asarray(bits, dtype=np.int8))
+        if b.ndim == 1:
+            b = b[None, :]
+        if b.shape[1] != self.ns:
+            raise ValueError(f"strings must have {self.ns} steps")
+        return b
+
+    def prob(self, bits):
+        b = self._bits(bits)
+        out = np.empty(b.shape[0])
+        self._check(library().gs_tt_prob(self._h, _i8(b), b.shape[0], _f8(out)))
+        return out
+
+    def chain(self, bits):
+        """``(log2 Pr, conditionals [nstr, ns], Z [nstr] complex)`` (``gsamp chain``)."""
+        b = self._bits(bits)
+        lp = np.empty(b.shape[0])
+        cond = np.empty((b.shape[0], self.ns))
+        amp = np.empty((b.shape[0], 2))
+        self._check(library().gs_tt_chain(self._h, _i8(b), b.shape[0], _f8(lp), _f8(cond), _f8(amp)))
+        return lp, cond, amp[:, 0] + 1j * amp[:, 1]
+
+    def sample(self, count, seed, first=0):
+        bits = np.empty((count, self.ns), np.int8)
+        lp = np.empty(count)
+        self._check(library().gs_tt_sample(self._h, ctypes.c_uint64(seed), first, count, _i8(bits), _f8(lp)))
+        return bits, lp
+
+    def hybrid(self, bits, seed, first_line=0):
+        b = self._bits(bits, allow_q=True)
+        ob = np.empty_like(b)
+        lp = np.empty(b.shape[0])
+        cond = np.empty((b.shape[0], self.ns))
+        self._check(library().gs_tt_hybrid(self._h, _i8(b), b.shape[0], ctypes.c_uint64(seed), first_line, _i8(ob),
+                                           _f8(lp), _f8(cond)))
+        return ob, lp, cond
+
+    def dp(self, jobs):
+        """jobs: [[(pre complex, [(w0, w1)] per vertex)] per job] -> complex [njobs] (``dp``)."""
+        nsl = np.array([len(j) for j in jobs], np.int32)
+        pre = np.array([[p.real, p.imag] for j in jobs for p, _ in j], np.float64).reshape(-1)
+        w = np.array([[[a.real, a.imag, c.real, c.imag] for a, c in ws] for j in jobs for _, ws in j],
+                     np.float64).reshape(-1)
+        out = np.empty(2 * len(jobs))
+        self._check(library().gs_tt_dp(self._h, len(jobs), nsl.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       _f8(np.ascontiguousarray(pre)), _f8(np.ascontiguousarray(w)), _f8(out)))
+        return out[0::2] + 1j * out[1::2]

@@ -111,6 +111,29 @@ int gs_batch_grad(gs_batch* batch, const int8_t* const* bits, double* const* log
 double gs_batch_grad_ms(const gs_batch* batch, double* forward_ms);
 const char* gs_batch_last_error(void);
 
+/* ---- two-table (gather-form) engine: the reference's gsamp.cpp sampler and dp.cpp amplitude engine.  Two tables of
+ * 2^maxr entries; step k's table is read (gathered) and the next one written (contiguous), gsamp.cpp:21-29. */
+typedef struct gs_tt_plan gs_tt_plan;
+typedef struct gs_tt_engine gs_tt_engine;
+/* gsamp plan (with_m = 1: per step `v r rn rho pivmask nker ker... M`, gsamp.cpp:96-107) or dp plan (with_m = 0,
+ * dp.cpp:16-25).  Exit codes: 2 header, 3 step head, 4 kernel vector, 5 M, 15 a step outside what the engine runs. */
+int gs_tt_plan_load(const char* path, int with_m, gs_tt_plan** out);
+int gs_tt_plan_info(const gs_tt_plan* plan, int* n, int* ns, int* maxr);
+void gs_tt_plan_destroy(gs_tt_plan* plan);
+int gs_tt_engine_create(const gs_tt_plan* plan, int storage, int device, gs_tt_engine** out);
+void gs_tt_engine_destroy(gs_tt_engine* eng);
+/* prob (gsamp.cpp:39-55 forced, both branch norms per step), chain (chain1, gsamp.cpp:59-70; amp_reim may be NULL),
+ * sample (mt19937_64(seed + first + i), gsamp.cpp:111-117), hybrid ('?' = -1 sampled, gsamp.cpp:118-123). */
+int gs_tt_prob(gs_tt_engine* eng, const int8_t* bits, int64_t nstr, double* log2p);
+int gs_tt_chain(gs_tt_engine* eng, const int8_t* bits, int64_t nstr, double* log2p, double* cond, double* amp_reim);
+int gs_tt_sample(gs_tt_engine* eng, uint64_t seed, int64_t first, int64_t count, int8_t* bits, double* log2p);
+int gs_tt_hybrid(gs_tt_engine* eng, const int8_t* bits, int64_t nstr, uint64_t seed, int64_t first_line,
+                 int8_t* out_bits, double* log2p, double* cond);
+/* dp (dp.cpp:33-62): out[j] = sum over job j's nslices[j] slices of pre * Z(w_slice); pre [total slices][2],
+ * w [total slices][m][4] (w0re w0im w1re w1im per vertex), out [njobs][2]. */
+int gs_tt_dp(gs_tt_engine* eng, int64_t njobs, const int* nslices, const double* pre, const double* w, double* out);
+const char* gs_tt_last_error(void);
+
 /* Timing of the device work of the last call, in milliseconds (HIP events around the step kernels). */
 double gs_last_device_ms(const gs_engine* eng);
 
@@ -122,10 +145,15 @@ int gs_profile_read(const gs_engine* eng, double* ms, int64_t* launches);
 /* Launch geometry of step k, out[8]: log2 cosets per 16-byte run, blocks, slots per coset, route, fusable, blocks
  * of the fused sweep, next step's route, log2 tiles per block. */
 int gs_step_geometry(const gs_engine* eng, int k, int* out);
+/* Coset-coordinate columns (physical address vectors) of step k as the plain (fused = 0) or fused sweep uses them,
+ * into cols[0 .. cap-1]; returns nk, or -1 when the step or layout does not exist. */
+int gs_step_columns(const gs_engine* eng, int k, int fused, uint64_t* cols, int cap);
 /* Fold the next step's norm pass into the previous write (on, default) or run it as its own sweep (off). */
 int gs_set_fusion(gs_engine* eng, int on);
 /* Cap one sweep at 2^lg blocks (1..14, default 9); tiles beyond the cap loop inside each block. */
 int gs_set_grid_log2(gs_engine* eng, int lg);
+/* Tile order of the sweeps: 0 forward (default), 1 reverse on odd steps, 2 reverse on every step. */
+int gs_set_walk(gs_engine* eng, int mode);
 /* Runs of two or more steps with at most 2^nk cosets go to one single-block launch (nk -1: never; default 10). */
 int gs_set_small_steps(gs_engine* eng, int nk);
 /* One plain device copy of `bytes` (read + write), *ms its duration: calibrates hardware byte counters. */

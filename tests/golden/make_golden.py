@@ -12,7 +12,13 @@ repository, the reference does not).
 * c1/     -- the n = 24, band 10 instance planned by this package (plan.txt), 64 uniform strings, 8 hybrid lines,
              and the reference executor's outputs for every mode in complex64 and complex128;
 * small/  -- 12 random small circuits (n <= 9) planned by this package with every string of each, and the
-             reference executor's prob / chain outputs (complex128).
+             reference executor's prob / chain outputs (complex128);
+* tt/     -- the two-table engine: the config-1 instance's gsamp plan (engine.write_sampler_plan) with the reference
+             gsamp's prob / chain / sample / hybrid outputs; a dp plan and sliced jobs (engine.write_plan,
+             engine.Sliced, engine.write_jobs) with the reference dp's outputs (complex128 and -DDP_FLOAT); and the
+             committed chain known answers: results/chain/gate_fwd_out.txt and the first 4 lines of
+             results/chain/ideal_fwd_chain.txt (strings from cloud_chain/ideal_fwd.txt) on cloud_ibmxeb/nq70_w30_plan.txt,
+             cloud_chain/plan_rev.txt with cloud_chain/gate_rev.txt and results/chain/gate_rev_out.txt.
 """
 import json
 import os
@@ -153,9 +159,59 @@ def small():
         json.dump(cases, f)
 
 
+def run_bin(name, args):
+    exe = os.path.join(REFBIN, name)
+    res = subprocess.run([exe] + args, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="4"))
+    if res.returncode:
+        raise RuntimeError(f"{exe} {args}: exit {res.returncode} {res.stderr}")
+    return res.stdout
+
+
+def twotable():
+    d = os.path.join(HERE, "tt")
+    os.makedirs(d, exist_ok=True)
+    inst = egs.instances.Instance(24, 10, 1)
+    plan = os.path.join(d, "c1_plan.txt")
+    egs.engine.write_sampler_plan(plan, inst.am)
+    strings = os.path.join(HERE, "c1", "strings.txt")
+    hybrid = os.path.join(HERE, "c1", "hybrid.txt")
+    for mode, args in (("prob", ["prob", strings]), ("chain", ["chain", strings]), ("sample", ["sample", "16", "1234"]),
+                       ("hybrid", ["hybrid", hybrid, "99"])):
+        with open(os.path.join(d, f"c1_{mode}.txt"), "w") as f:
+            f.write(run_bin("gsamp", [plan] + args))
+    # dp: the graph with vertices 3 and 11 sliced, weights of 6 of the strings (w_q = M_q[b_q, :])
+    A, M = inst.am.A, inst.am.M
+    sl = egs.engine.Sliced(A, inst.order, [3, 11])
+    egs.engine.write_plan(os.path.join(d, "dp_plan.txt"), sl.steps, sl.m)
+    bits = egs.planio.read_strings(strings, len(inst.steps))[:6]
+    ws = []
+    for b in bits:
+        bq = egs.planio.step_to_qubit(b, inst.step_vertices, inst.N)
+        ws.append([(complex(M[q][bq[q], 0]), complex(M[q][bq[q], 1])) for q in range(inst.N)])
+    egs.engine.write_jobs(os.path.join(d, "dp_jobs.txt"), sl, ws)
+    for name, tag in (("dp", "c128"), ("dp_float", "c64")):
+        with open(os.path.join(d, f"dp_out_{tag}.txt"), "w") as f:
+            f.write(run_bin(name, [os.path.join(d, "dp_plan.txt"), os.path.join(d, "dp_jobs.txt")]))
+    # committed known answers of the reference's two-table chain mode
+    with open(os.path.join(REF, "results/chain/gate_fwd_out.txt")) as f:
+        open(os.path.join(d, "ibm_gate_fwd_chain.txt"), "w").write(f.read())
+    with open(os.path.join(REF, "cloud_chain/ideal_fwd.txt")) as f:
+        open(os.path.join(d, "ibm_ideal_fwd_strings.txt"), "w").write("".join(f.readlines()[:4]))
+    with open(os.path.join(REF, "results/chain/ideal_fwd_chain.txt")) as f:
+        open(os.path.join(d, "ibm_ideal_fwd_chain.txt"), "w").write("".join(f.readlines()[:4]))
+    copy("cloud_chain/plan_rev.txt", os.path.join(d, "ibm_plan_rev.txt"))
+    copy("cloud_chain/gate_rev.txt", os.path.join(d, "ibm_gate_rev.txt"))
+    copy("results/chain/gate_rev_out.txt", os.path.join(d, "ibm_gate_rev_chain.txt"))
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1:
+        for name in sys.argv[1:]:
+            globals()[name]()
+        sys.exit(0)
     ibm()
     doped()
     c1()
     small()
+    twotable()
     print("fixtures written under", HERE)

@@ -238,19 +238,21 @@ def c4_plan(tmp_path_factory):
     return path
 
 
-@pytest.mark.parametrize("grid_log2", [14, 9, 6, 4])
-def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2):
+@pytest.mark.parametrize("grid_log2,walk", [(14, 0), (9, 0), (9, 1), (6, 2), (4, 0)])
+def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2, walk):
     """n=60 band-24 instance (2^20 arena, up to 2^10 blocks per sweep): samples from the FP64 arena equal the
     oracle's bits and log2 Pr (1e-12) whatever the block cap -- one tile per block with a separate reduce kernel,
-    512 looping blocks with the reduce kernel, 64 or 16 looping blocks reducing in their last block."""
+    512 looping blocks with the reduce kernel, 64 or 16 looping blocks reducing in their last block -- and whatever the
+    tile order (forward, reversed on odd steps, reversed everywhere)."""
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     import subprocess
     out = subprocess.run([os.path.join(root, "oracle", "gs_oracle"), "--c128", c4_plan, "sample", "3", "777"],
                          capture_output=True, text=True, check=True).stdout.split("\n")
     ref = [l.split() for l in out if l.strip()]
-    print(f"c4 sample 3 777 grid_log2={grid_log2}", flush=True)
+    print(f"c4 sample 3 777 grid_log2={grid_log2} walk={walk}", flush=True)
     with gs.Engine(c4_plan, gs.C128) as e:
         e.set_grid_log2(grid_log2)
+        e.set_walk(walk)
         assert max(e.geometry(k)["grid"] for k in range(e.ns)) <= 1 << grid_log2
         bits, lp = e.sample(3, 777)
         lpp = e.prob(bits)

@@ -1,0 +1,97 @@
+"""The two-table (gather-form) engine against the reference gsamp / dp outputs (tests/golden/tt, complex64 tables as
+the reference's; dp complex128 and -DDP_FLOAT), and the reference's committed two-table chain known answers on IBM's
+width-30 plans (results/chain/gate_fwd_out.txt, gate_rev_out.txt, ideal_fwd_chain.txt)."""
+import importlib.util
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from _pkg import egs
+
+gs = egs.gsamp
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+spec = importlib.util.spec_from_file_location("twotable_ref", os.path.join(ROOT, "oracle", "twotable_ref.py"))
+tt = importlib.util.module_from_spec(spec)
+spec.loader.exec_module(tt)
+
+
+def lines(path):
+    return [l.split() for l in open(path) if l.strip()]
+
+
+@pytest.fixture(scope="module")
+def tdir(golden):
+    return os.path.join(golden, "tt")
+
+
+def test_c1_modes_vs_reference_gsamp(tdir, golden):
+    plan = os.path.join(tdir, "c1_plan.txt")
+    print("two-table c1: prob, chain, sample, hybrid", flush=True)
+    with gs.TwoTableEngine(plan) as e:
+        strings = egs.planio.read_strings(os.path.join(golden, "c1", "strings.txt"), e.ns)
+        hy = egs.planio.read_strings(os.path.join(golden, "c1", "hybrid.txt"), e.ns, allow_q=True)
+        lp = e.prob(strings)
+        lpc, cond, amp = e.chain(strings)
+        bits, lps = e.sample(16, 1234)
+        ob, lph, condh = e.hybrid(hy, 99)
+    assert np.max(np.abs(lp - [float(l[0]) for l in lines(os.path.join(tdir, "c1_prob.txt"))])) < 1e-5
+    refc = lines(os.path.join(tdir, "c1_chain.txt"))
+    assert np.max(np.abs(lpc - [float(l[0]) for l in refc])) < 1e-5
+    assert np.max(np.abs(cond - [[float(x) for x in l[1:]] for l in refc])) < 1e-5
+    assert np.max(np.abs(np.log2(np.abs(amp) ** 2) - e.ns - lpc)) < 1e-5
+    refs = lines(os.path.join(tdir, "c1_sample.txt"))
+    assert [egs.planio.format_bits(b) for b in bits] == [l[0] for l in refs]
+    assert np.max(np.abs(lps - [float(l[1]) for l in refs])) < 1e-5
+    refh = lines(os.path.join(tdir, "c1_hybrid.txt"))
+    assert [egs.planio.format_bits(b) for b in ob] == [l[0] for l in refh]
+    assert np.max(np.abs(lph - [float(l[1]) for l in refh])) < 1e-5
+
+
+@pytest.mark.parametrize("storage,tag,tol", [(gs.C128, "c128", 1e-12), (gs.C64, "c64", 1e-5)])
+def test_dp_vs_reference(tdir, storage, tag, tol):
+    m, _, _ = tt.load_plan(os.path.join(tdir, "dp_plan.txt"), with_m=False)
+    jobs = tt.read_jobs(os.path.join(tdir, "dp_jobs.txt"), m)
+    want = np.array([complex(float(a), float(b)) for a, b in lines(os.path.join(tdir, f"dp_out_{tag}.txt"))])
+    print(f"dp {tag}", flush=True)
+    with gs.TwoTableEngine(os.path.join(tdir, "dp_plan.txt"), with_m=False, storage=storage) as e:
+        got = e.dp(jobs)
+    assert np.max(np.abs(got - want)) < tol * np.max(np.abs(want))
+
+
+def test_ibm_chain_known_answers(tdir, golden):
+    """gsamp chain on cloud_ibmxeb/nq70_w30_plan.txt: the gate strings (results/chain/gate_fwd_out.txt) and 4 of the
+    1,012 ideal samples with all 70 conditionals (results/chain/ideal_fwd_chain.txt); the reversed-order plan
+    cloud_chain/plan_rev.txt with its gate strings (results/chain/gate_rev_out.txt)."""
+    print("two-table IBM w30 chain KATs", flush=True)
+    with gs.TwoTableEngine(os.path.join(golden, "ibm", "nq70_w30_plan.txt")) as e:
+        gate = egs.planio.read_strings(os.path.join(golden, "ibm", "gate_strings.txt"), e.ns)
+        lp, _, _ = e.chain(gate)
+        ideal = egs.planio.read_strings(os.path.join(tdir, "ibm_ideal_fwd_strings.txt"), e.ns)
+        lpi, condi, _ = e.chain(ideal)
+    want = [float(l[0]) for l in lines(os.path.join(tdir, "ibm_gate_fwd_chain.txt"))]
+    assert np.max(np.abs(lp - want)) < 1e-5, (lp, want)
+    ref = lines(os.path.join(tdir, "ibm_ideal_fwd_chain.txt"))
+    assert np.max(np.abs(lpi - [float(l[0]) for l in ref])) < 1e-5
+    assert np.max(np.abs(condi - [[float(x) for x in l[1:]] for l in ref])) < 1e-6
+    with gs.TwoTableEngine(os.path.join(tdir, "ibm_plan_rev.txt")) as e:
+        gate_rev = egs.planio.read_strings(os.path.join(tdir, "ibm_gate_rev.txt"), e.ns)
+        lpr, _, _ = e.chain(gate_rev)
+    want = [float(l[0]) for l in lines(os.path.join(tdir, "ibm_gate_rev_chain.txt"))]
+    assert np.max(np.abs(lpr - want)) < 1e-5, (lpr, want)
+
+
+def test_cli_front_ends(tdir, golden, tmp_path):
+    """gsamp_amd and dp_amd print the reference's lines."""
+    bindir = os.path.join(ROOT, "exact-graphstate-sampling_amd", "bin")
+    out = subprocess.run([os.path.join(bindir, "gsamp_amd"), os.path.join(tdir, "c1_plan.txt"), "sample", "3", "1234"],
+                         capture_output=True, text=True, check=True).stdout.splitlines()
+    ref = [l for l in open(os.path.join(tdir, "c1_sample.txt")).read().splitlines()][:3]
+    assert [l.split()[0] for l in out] == [l.split()[0] for l in ref]
+    assert [l.split()[2] for l in out] == [l.split()[2] for l in ref]
+    out = subprocess.run([os.path.join(bindir, "dp_amd"), os.path.join(tdir, "dp_plan.txt"),
+                          os.path.join(tdir, "dp_jobs.txt")], capture_output=True, text=True, check=True).stdout.split()
+    want = [float(x) for x in open(os.path.join(tdir, "dp_out_c128.txt")).read().split()]
+    assert np.max(np.abs(np.array([float(x) for x in out]) - want)) < 1e-12 * np.max(np.abs(want))

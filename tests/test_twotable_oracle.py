@@ -1,0 +1,59 @@
+"""The two-table restatement (oracle/twotable_ref.py) against the reference gsamp / dp outputs committed as fixtures
+(tests/golden/tt, written by the reference binaries compiled from their sources), and the independent cross-check the
+reference makes (results/xcheck70_independent.txt): dp's sliced complex amplitude equals the in-place planner's
+Z(w_b), whose |Z|^2 2^-ns is the reference gsamp2's probability."""
+import importlib.util
+import os
+
+import numpy as np
+import pytest
+
+from _pkg import egs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+spec = importlib.util.spec_from_file_location("twotable_ref", os.path.join(ROOT, "oracle", "twotable_ref.py"))
+tt = importlib.util.module_from_spec(spec)
+spec.loader.exec_module(tt)
+
+
+@pytest.fixture(scope="module")
+def tdir(golden):
+    return os.path.join(golden, "tt")
+
+
+def lines(path):
+    return [l.split() for l in open(path) if l.strip()]
+
+
+def test_prob_and_chain_match_reference_gsamp(tdir, golden):
+    _, _, steps = tt.load_plan(os.path.join(tdir, "c1_plan.txt"))
+    strings = egs.planio.read_strings(os.path.join(golden, "c1", "strings.txt"), len(steps))[:16]
+    ref_p = [float(l[0]) for l in lines(os.path.join(tdir, "c1_prob.txt"))]
+    ref_c = lines(os.path.join(tdir, "c1_chain.txt"))
+    for i, b in enumerate(strings):
+        assert abs(tt.prob(steps, b) - ref_p[i]) < 1e-9
+        lp, cond = tt.chain1(steps, b)
+        assert abs(lp - float(ref_c[i][0])) < 1e-9
+        assert np.max(np.abs(cond - np.array([float(x) for x in ref_c[i][1:]]))) < 2e-9
+
+
+def test_two_table_equals_in_place(tdir, golden):
+    """Two algorithms, one answer: the two-table chain of the config-1 plan equals the in-place executor's (reference
+    gsamp2 outputs) within complex64 rounding."""
+    ref_tt = [float(l[0]) for l in lines(os.path.join(tdir, "c1_prob.txt"))]
+    ref_ip = [float(l[0]) for l in lines(os.path.join(golden, "c1", "prob_c128.txt"))]
+    assert np.max(np.abs(np.array(ref_tt) - np.array(ref_ip))) < 1e-5
+
+
+def test_dp_matches_reference_and_in_place_amplitude(tdir, golden):
+    m, _, steps = tt.load_plan(os.path.join(tdir, "dp_plan.txt"), with_m=False)
+    jobs = tt.read_jobs(os.path.join(tdir, "dp_jobs.txt"), m)
+    want = np.array([complex(float(a), float(b)) for a, b in lines(os.path.join(tdir, "dp_out_c128.txt"))])
+    got = tt.dp(steps, jobs)
+    assert np.max(np.abs(got - want)) < 1e-12 * np.max(np.abs(want))
+    want32 = np.array([complex(float(a), float(b)) for a, b in lines(os.path.join(tdir, "dp_out_c64.txt"))])
+    assert np.max(np.abs(tt.dp(steps, jobs, c64=True) - want32)) < 1e-5 * np.max(np.abs(want))
+    # |Z|^2 2^-ns = Pr(b) from the in-place executor (reference gsamp2, complex128) for the same six strings
+    ns = 24
+    ref_ip = np.array([float(l[0]) for l in lines(os.path.join(golden, "c1", "prob_c128.txt"))][:6])
+    assert np.max(np.abs(np.log2(np.abs(want) ** 2) - ns - ref_ip)) < 1e-9
