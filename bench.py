@@ -57,6 +57,7 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=8, help="samples in the bounded CPU baseline run")
     ap.add_argument("--c128-steps", type=int, default=16)
     ap.add_argument("--ibm-steps", type=int, default=8, help="samples of the IBM width-30 plan (0: skip)")
+    ap.add_argument("--tt-steps", type=int, default=2, help="samples of the two-table width-31 plan (0: skip)")
     ap.add_argument("--c3-strings", type=int, default=8192)
     ap.add_argument("--c3-lanes", type=int, default=512)
     ap.add_argument("--c3-single", type=int, default=256, help="strings timed one per launch sequence")
@@ -162,6 +163,7 @@ def main():
     same_bits = int(sum(int(np.array_equal(a, b)) for a, b in zip(bits[:K2], bits2)))
 
     ibm = ibm_w30(gs, egs, args.ibm_steps, args) if args.ibm_steps else None
+    tt2 = ibm_two_table(gs, egs, args.tt_steps) if args.tt_steps else None
 
     cpu = None
     if not args.no_cpu:
@@ -200,6 +202,7 @@ def main():
                         "max_abs_dlog2p": dlog, "samples_compared": K2, "identical_bits": same_bits},
         "cpu_baseline": cpu,
         "ibm_w30": ibm,
+        "ibm_two_table": tt2,
     }
     print(json.dumps(out))
 
@@ -395,6 +398,33 @@ def ibm_w30(gs, egs, K, args):
             "reference_published_hardware": "laptop, AMD Ryzen 9 9955HX (README.md:24)",
             "speedup_vs_published": IBM_PUBLISHED_S / wall,
             "first_sample": [egs.planio.format_bits(bits[0]), float(lp[0])]}
+
+
+def ibm_two_table(gs, egs, K):
+    """The two-table engine on the reference's committed two-table plans of the IBM circuit: exact samples of the
+    width-31 plan (published 13.0 s per sample, README.md:24, 2 x 2^31 complex64 tables) and prob of the two gate
+    strings on the width-30 plan (published 6.9 s per string, README.md:138-140)."""
+    d = os.path.join(ROOT, "tests", "golden", "ibm")
+    out = {}
+    with gs.TwoTableEngine(os.path.join(d, "nq70_w31_plan.txt")) as e:
+        e.sample(1, 7)
+        t0 = time.perf_counter()
+        e.sample(K, 1000)
+        ts = (time.perf_counter() - t0) / K
+    out["w31_sample"] = {"s_per_sample": ts, "samples": K, "reference_published_s": 13.0,
+                         "speedup_vs_published": 13.0 / ts}
+    with gs.TwoTableEngine(os.path.join(d, "nq70_w30_plan.txt")) as e:
+        strings = egs.planio.read_strings(os.path.join(d, "gate_strings.txt"), e.ns)
+        e.prob(strings[:1])
+        t0 = time.perf_counter()
+        e.prob(strings)
+        tp = (time.perf_counter() - t0) / len(strings)
+        t0 = time.perf_counter()
+        e.chain(strings)
+        tc = (time.perf_counter() - t0) / len(strings)
+    out["w30_prob"] = {"s_per_string": tp, "reference_published_s": 6.9, "speedup_vs_published": 6.9 / tp}
+    out["w30_chain"] = {"s_per_string": tc}
+    return out
 
 
 TRAFFIC_PROFILE = "profiles/sweep_c2_hbm_traffic.md"
