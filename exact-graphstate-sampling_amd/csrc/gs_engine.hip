@@ -249,6 +249,7 @@ struct gs_engine {
     int grid_cap = kDefaultGridLog2;  // log2 of the most blocks a sweep launches
     int small_nk = 10;              // runs of steps with nk <= small_nk go to one single-block launch (-1: never)
     int walk = 0;                   // tile order: 0 forward, 1 reverse on odd steps, 2 reverse on every step
+    int fused_run = -1;             // log2 run of the fused layout (-1: the storage's default)
     // optional per-sweep timing (bench / profiling)
     bool profile = false;
     std::vector<hipEvent_t> pev;
@@ -527,7 +528,7 @@ bool build_fused(const gs_engine* e, int k, KStep& f) {
     const KStep& p0 = e->hplain[size_t(k)];
     const KStep& p1 = e->hplain[size_t(k + 1)];
     if (p1.ntq != 2 || p1.growth) return false;
-    const int nk = s0.nk, lrf = fused_lr(e->storage);
+    const int nk = s0.nk, lrf = e->fused_run >= 0 ? e->fused_run : fused_lr(e->storage);
     // coordinates of the first lrf unit physical bits (one aligned run per thread)
     std::vector<uint64_t> lead;
     for (int i = 0; i < lrf; i++) {
@@ -1167,8 +1168,8 @@ int gs_profile_read(const gs_engine* eng, double* ms, int64_t* launches) {
     return 0;
 }
 
-// Launch geometry of step k, out[8]: plain log2 run, plain blocks, ntq, route, fusable, fused blocks, next route,
-// plain log2 tiles per block.
+// Launch geometry of step k, out[10]: plain log2 run, plain blocks, ntq, route, fusable, fused blocks, next route,
+// plain log2 tiles per block, fused log2 run and fused log2 tiles per block (-1 when the step does not fuse).
 int gs_step_geometry(const gs_engine* eng, int k, int* out) {
     if (!eng || k < 0 || k >= eng->plan.ns || !out) return fail(16, "bad step");
     const KStep& s = eng->hplain[size_t(k)];
@@ -1181,6 +1182,8 @@ int gs_step_geometry(const gs_engine* eng, int k, int* out) {
     out[5] = eng->fusable[size_t(k)] ? 1 << f.lg_grid : 0;
     out[6] = eng->fusable[size_t(k)] ? f.route2 : 0;
     out[7] = s.lg_iter;
+    out[8] = eng->fusable[size_t(k)] ? f.lr : -1;
+    out[9] = eng->fusable[size_t(k)] ? f.lg_iter : -1;
     return 0;
 }
 
@@ -1210,6 +1213,23 @@ int gs_step_columns(const gs_engine* eng, int k, int fused, uint64_t* cols, int 
     const KStep& s = fused ? eng->hfused[size_t(k)] : eng->hplain[size_t(k)];
     for (int i = 0; i < s.nk && i < cap; i++) cols[i] = s.cols[i];
     return s.nk;
+}
+
+// log2 run length of the fused layout (complex64: 1 or 2; complex128: 0; -1 restores the default).  Rebuilds the
+// step descriptors.
+int gs_set_fused_run(gs_engine* eng, int lr) {
+    if (!eng) return fail(16, "null engine");
+    if (lr < -1 || lr > (eng->storage == GS_C64 ? 2 : 0)) return fail(16, "run length not available for this storage");
+    HIP_TRY(hipSetDevice(eng->device));
+    HIP_TRY(hipStreamSynchronize(eng->stream));
+    eng->fused_run = lr;
+    if (int rc = build_steps(eng)) return rc;
+    const size_t ns = size_t(eng->plan.ns);
+    if (ns) {
+        HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+    }
+    return 0;
 }
 
 // Tile order of the sweeps: 0 forward, 1 reverse on odd steps, 2 reverse on every step (results do not depend on it).

@@ -871,7 +871,11 @@ void* by_layout(int lr, int vg, int ntq, int route, int route2) {
     constexpr bool c64 = sizeof(S) == 8;
     if constexpr (SWEEP == kWriteGram || SWEEP == kWriteDirect) {
         if (vg != 2) return nullptr;
-        if constexpr (c64) return lr == 1 ? by_ntq<S, 2, 2, SWEEP>(ntq, route, route2) : nullptr;
+        if constexpr (c64) {
+            if (lr == 1) return by_ntq<S, 2, 2, SWEEP>(ntq, route, route2);
+            if (lr == 2) return by_ntq<S, 4, 2, SWEEP>(ntq, route, route2);
+            return nullptr;
+        }
         else return lr == 0 ? by_ntq<S, 1, 2, SWEEP>(ntq, route, route2) : nullptr;
     } else {
         if (vg != 1) return nullptr;

@@ -21,7 +21,7 @@ SEED_PLAIN, SEED_SEQ = 0, 1
 ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy", "gs_engine_create",
                "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
                "gs_last_device_ms", "gs_last_error", "gs_profile_enable", "gs_profile_read",
-               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_small_steps", "gs_set_walk", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
+               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_small_steps", "gs_set_walk", "gs_set_fused_run", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
                "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms", "gs_tt_plan_load", "gs_tt_plan_info",
                "gs_tt_plan_destroy", "gs_tt_engine_create", "gs_tt_engine_destroy", "gs_tt_prob", "gs_tt_chain",
@@ -75,6 +75,7 @@ def library():
         "gs_set_grid_log2": (I, [P, I]),
         "gs_set_small_steps": (I, [P, I]),
         "gs_set_walk": (I, [P, I]),
+        "gs_set_fused_run": (I, [P, I]),
         "gs_copy_probe": (I, [I, I64, dp]),
         "gs_batch_create": (I, [ctypes.POINTER(P), ip, I, I, I, pp]),
         "gs_batch_set_matrices": (I, [P, I, dp]),
@@ -271,9 +272,10 @@ class Engine:
 
     def geometry(self, k):
         """Launch geometry of step k: dict(lr, grid, ntq, route, fusable, fgrid, route2, lg_iter)."""
-        out = np.zeros(8, np.int32)
+        out = np.zeros(10, np.int32)
         _check(library().gs_step_geometry(self._h, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
-        return dict(zip(("lr", "grid", "ntq", "route", "fusable", "fgrid", "route2", "lg_iter"), (int(x) for x in out)))
+        return dict(zip(("lr", "grid", "ntq", "route", "fusable", "fgrid", "route2", "lg_iter", "flr", "flg_iter"),
+                        (int(x) for x in out)))
 
     def columns(self, k, fused=False):
         """Coset-coordinate columns of step k as the plain or fused sweep uses them (None: no such layout)."""
@@ -284,6 +286,10 @@ class Engine:
     def set_fusion(self, on):
         """Fold each sampled step's norm pass into the previous write sweep where possible (default on)."""
         _check(library().gs_set_fusion(self._h, 1 if on else 0))
+
+    def set_fused_run(self, lr):
+        """log2 run length of the fused layout (complex64: 1 or 2; -1: default)."""
+        _check(library().gs_set_fused_run(self._h, int(lr)))
 
     def set_walk(self, mode):
         """Tile order of the sweeps: 0 forward, 1 reverse on odd steps, 2 reverse on every step."""

@@ -33,7 +33,7 @@ def close(a, b, tol):
     return float(np.max(d)) if d.size else 0.0
 
 
-PATHS = ["small-runs", "fused", "plain"]
+PATHS = ["small-runs", "fused", "fused-run4", "plain"]
 
 
 def configure(e, path):
@@ -41,6 +41,8 @@ def configure(e, path):
     its own sweeps, norms folded into the previous write; plain: one norm sweep per sampled step."""
     e.set_small_steps(10 if path == "small-runs" else -1)
     e.set_fusion(path != "plain")
+    if path == "fused-run4" and e.storage == gs.C64:
+        e.set_fused_run(2)
 
 
 @pytest.fixture(scope="module")
@@ -238,8 +240,8 @@ def c4_plan(tmp_path_factory):
     return path
 
 
-@pytest.mark.parametrize("grid_log2,walk", [(14, 0), (9, 0), (9, 1), (6, 2), (4, 0)])
-def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2, walk):
+@pytest.mark.parametrize("grid_log2,walk,run", [(14, 0, -1), (9, 0, -1), (9, 1, -1), (6, 2, -1), (4, 0, -1), (9, 0, 2)])
+def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2, walk, run):
     """n=60 band-24 instance (2^20 arena, up to 2^10 blocks per sweep): samples from the FP64 arena equal the
     oracle's bits and log2 Pr (1e-12) whatever the block cap -- one tile per block with a separate reduce kernel,
     512 looping blocks with the reduce kernel, 64 or 16 looping blocks reducing in their last block -- and whatever the
@@ -249,16 +251,21 @@ def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2, walk):
     out = subprocess.run([os.path.join(root, "oracle", "gs_oracle"), "--c128", c4_plan, "sample", "3", "777"],
                          capture_output=True, text=True, check=True).stdout.split("\n")
     ref = [l.split() for l in out if l.strip()]
-    print(f"c4 sample 3 777 grid_log2={grid_log2} walk={walk}", flush=True)
-    with gs.Engine(c4_plan, gs.C128) as e:
+    print(f"c4 sample 3 777 grid_log2={grid_log2} walk={walk} run={run}", flush=True)
+    storage = gs.C64 if run == 2 else gs.C128        # the 4-run fused layout is a complex64 layout
+    with gs.Engine(c4_plan, storage) as e:
         e.set_grid_log2(grid_log2)
         e.set_walk(walk)
+        e.set_fused_run(run)
         assert max(e.geometry(k)["grid"] for k in range(e.ns)) <= 1 << grid_log2
+        if run == 2:
+            assert any(e.geometry(k)["flr"] == 2 for k in range(e.ns))
         bits, lp = e.sample(3, 777)
         lpp = e.prob(bits)
+    tol = TOL[storage]
     assert [egs.planio.format_bits(b) for b in bits] == [r[0] for r in ref]
-    assert close(lp, [float(r[1]) for r in ref], 0) < 1e-12
-    assert close(lpp, lp, 0) < 1e-12
+    assert close(lp, [float(r[1]) for r in ref], 0) < tol
+    assert close(lpp, lp, 0) < tol
 
 
 @pytest.mark.parametrize("storage", [gs.C64, gs.C128])
