@@ -1,10 +1,12 @@
 // gsamp_amd / dp_amd -- command-line front ends of the two-table engine with the reference's contracts:
 //   gsamp_amd [--c128] [--device N] PLAN sample N SEED | prob STRINGS | chain STRINGS | hybrid STRINGS SEED
+//                                        | prefix STRINGS K OUTPREFIX | resume STRINGS K SEED DUMPPREFIX...
 //                                                                        (gsamp.cpp:1-5, :108-161)
 //   dp_amd    [--c64] [--device N] PLAN JOBS                             (dp.cpp:1-4, :26-62)
 // The program's file name selects the contract (a name ending in "dp_amd" is the amplitude engine).  Output lines and
 // exit codes are the reference's; gsamp's tables are complex64 by default (the reference's only storage), dp's
 // complex128 (the reference's default; --c64 is its -DDP_FLOAT build).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -160,8 +162,94 @@ int main(int argc, char** argv) {
                 for (int k = 0; k < ns; k++) printf(" %.9f", cond[size_t(i) * ns + k]);
                 putchar('\n');
             }
+    } else if (mode == "prefix") {
+        // gsamp.cpp:124-135: lines of at least K characters; the table read by step K to OUTPREFIX_<line>.bin
+        if (argc < 6) return 1;
+        const int K = atoi(argv[4]);
+        if (K < 0 || K > ns) return 16;
+        std::vector<int8_t> pre;
+        int64_t cnt = 0;
+        if ((rc = read_strings(argv[3], K, false, pre, cnt))) return rc == 16 ? 8 : rc;
+        int r = 0;
+        std::vector<int8_t> bits(static_cast<size_t>(ns), 0);
+        if ((rc = gs_tt_prefix(eng, bits.data(), K, nullptr, 0, &r, nullptr, nullptr))) return die(rc);
+        const int64_t nt = int64_t(1) << r;
+        std::vector<double> tab(2 * static_cast<size_t>(nt));
+        std::vector<float> tf(2 * static_cast<size_t>(nt));
+        for (int64_t i = 0; i < cnt && !rc; i++) {
+            std::copy(pre.begin() + i * K, pre.begin() + (i + 1) * K, bits.begin());
+            double ls = 0, lp = 0;
+            if ((rc = gs_tt_prefix(eng, bits.data(), K, tab.data(), nt, &r, &ls, &lp))) break;
+            const std::string fn = std::string(argv[5]) + "_" + std::to_string(i) + ".bin";
+            FILE* o = fopen(fn.c_str(), "wb");
+            if (!o) return 9;
+            const int one = 1;
+            for (size_t q = 0; q < tf.size(); q++) tf[q] = float(tab[q]);
+            fwrite(&r, 4, 1, o);
+            fwrite(&one, 4, 1, o);
+            fwrite(&ls, 8, 1, o);
+            fwrite(tf.data(), sizeof(float), tf.size(), o);
+            fclose(o);
+            printf("%lld %.12f %.6f\n", (long long)i, lp, ls);
+            fflush(stdout);
+        }
+    } else if (mode == "resume") {
+        // gsamp.cpp:136-148: per line, sum the dumps DUMPPREFIX_s_<line>.bin and continue steps K..ns-1
+        if (argc < 7) return 1;
+        const int K = atoi(argv[4]);
+        const unsigned long long seed = strtoull(argv[5], nullptr, 10);
+        const int nsl = argc - 6;
+        if (K < 0 || K > ns) return 16;
+        std::vector<int8_t> bits;
+        int64_t cnt = 0;
+        if ((rc = read_strings(argv[3], ns, true, bits, cnt))) return rc;
+        int r = 0;                                         // st[K].r: every dump must match it
+        const std::vector<int8_t> zeros(static_cast<size_t>(ns), 0);
+        if ((rc = gs_tt_prefix(eng, zeros.data(), K, nullptr, 0, &r, nullptr, nullptr))) return die(rc);
+        std::vector<int8_t> ob(static_cast<size_t>(ns));
+        std::vector<double> cond(static_cast<size_t>(ns));
+        for (int64_t i = 0; i < cnt && !rc; i++) {
+            std::vector<std::vector<double>> tabs(static_cast<size_t>(nsl));
+            std::vector<const double*> ptrs(static_cast<size_t>(nsl));
+            std::vector<double> ls(static_cast<size_t>(nsl));
+            for (int s = 0; s < nsl; s++) {
+                const std::string fn = std::string(argv[6 + s]) + "_" + std::to_string(i) + ".bin";
+                FILE* o = fopen(fn.c_str(), "rb");
+                if (!o) {
+                    fprintf(stderr, "missing %s\n", fn.c_str());
+                    return 9;
+                }
+                int r2 = 0, one = 0;
+                if (fread(&r2, 4, 1, o) != 1 || fread(&one, 4, 1, o) != 1 || fread(&ls[size_t(s)], 8, 1, o) != 1 ||
+                    r2 != r) {
+                    fprintf(stderr, "bad dump %s\n", fn.c_str());
+                    fclose(o);
+                    return 10;
+                }
+                const size_t nt = size_t(1) << r2;
+                std::vector<float> tf(2 * nt);
+                if (fread(tf.data(), sizeof(float), tf.size(), o) != tf.size()) {
+                    fprintf(stderr, "bad dump %s\n", fn.c_str());
+                    fclose(o);
+                    return 10;
+                }
+                fclose(o);
+                tabs[size_t(s)].assign(tf.begin(), tf.end());
+                ptrs[size_t(s)] = tabs[size_t(s)].data();
+            }
+            double lpf = 0, lpp = 0;
+            const int8_t* row = bits.data() + i * ns;
+            if ((rc = gs_tt_resume(eng, row, K, nsl, ptrs.data(), ls.data(), seed, i, ob.data(), &lpf, &lpp,
+                                   cond.data())))
+                break;
+            for (int k = 0; k < ns; k++) putchar('0' + ob[size_t(k)]);
+            printf(" %.12f %.12f", lpf, lpp);
+            for (int k = K; k < ns; k++) printf(" %.9f", cond[size_t(k)]);
+            putchar('\n');
+            fflush(stdout);
+        }
     } else {
-        fprintf(stderr, "mode must be sample, prob, chain or hybrid\n");
+        fprintf(stderr, "mode must be sample, prob, chain, hybrid, prefix or resume\n");
         rc = 16;
     }
     gs_tt_engine_destroy(eng);

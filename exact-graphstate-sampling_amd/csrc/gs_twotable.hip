@@ -490,6 +490,101 @@ int gs_tt_hybrid(gs_tt_engine* e, const int8_t* bits, int64_t nstr, uint64_t see
     return 0;
 }
 
+// prefix (gsamp.cpp:124-135): forced steps 0..K-1 of one string (run_steps semantics), then the contiguous table read
+// by step K (2^r entries, r = st[K].r; K == ns: the amplitude) as complex doubles, its logscale and log2 Pr(prefix).
+// table_reim = NULL: only *r_out is set (no steps run).
+int gs_tt_prefix(gs_tt_engine* e, const int8_t* bits, int K, double* table_reim, int64_t table_cap, int* r_out,
+                 double* logscale, double* log2p_prefix) {
+    if (!e || !e->plan.with_m || !bits || !r_out || K < 0 || K > e->plan.ns) return ttfail(16, "bad argument");
+    const int ns = e->plan.ns;
+    for (int k = 0; k < K; k++)
+        if (bits[k] != 0 && bits[k] != 1) return ttfail(8, "'?' before K");
+    const int r = K < ns ? e->plan.steps[size_t(K)].r : 0;
+    *r_out = r;
+    if (!table_reim) return 0;
+    const int64_t n = int64_t(1) << r;
+    if (table_cap < n || !logscale || !log2p_prefix) return ttfail(16, "table capacity below 2^r");
+    TT_TRY(hipSetDevice(e->device));
+    memcpy(e->h_forced, bits, size_t(ns));
+    TT_TRY(hipMemcpyAsync(e->d_forced, e->h_forced, size_t(ns), hipMemcpyHostToDevice, e->stream));
+    if (int rc = tt_init(e, 1.0, 0.0)) return rc;
+    if (int rc = tt_enqueue(e, kRun, 0, K, e->h_forced, e->d_forced, e->d_U, e->d_bits, e->d_cond, e->d_out, nullptr))
+        return rc;
+    std::vector<unsigned char> raw(size_t(n) * e->esz);
+    TT_TRY(hipMemcpyAsync(raw.data(), e->tab[e->cur], raw.size(), hipMemcpyDeviceToHost, e->stream));
+    TT_TRY(hipMemcpyAsync(e->h_out, e->d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    TT_TRY(hipStreamSynchronize(e->stream));
+    for (int64_t i = 0; i < n; i++) {
+        if (e->storage == GS_C64) {
+            float v[2];
+            memcpy(v, raw.data() + size_t(i) * 8, 8);
+            table_reim[2 * i] = v[0];
+            table_reim[2 * i + 1] = v[1];
+        } else {
+            memcpy(table_reim + 2 * i, raw.data() + size_t(i) * 16, 16);
+        }
+    }
+    *log2p_prefix = e->h_out[0];
+    *logscale = e->h_out[1];
+    return 0;
+}
+
+// resume (gsamp.cpp:136-148) for one line: the slices' step-K tables summed with weights 2^(ls_s - max ls) into the
+// first table (stored in the table type), then steps K..ns-1 with '?' (-1) sampled from mt19937_64(seed + line).
+int gs_tt_resume(gs_tt_engine* e, const int8_t* bits, int K, int nslices, const double* const* tables,
+                 const double* logscales, uint64_t seed, int64_t line, int8_t* out_bits, double* log2p_full,
+                 double* log2p_prefix, double* cond) {
+    if (!e || !e->plan.with_m || !bits || K < 0 || K > e->plan.ns || nslices <= 0 || !tables || !logscales ||
+        !out_bits || !log2p_full || !log2p_prefix || !cond)
+        return ttfail(16, "bad argument");
+    const int ns = e->plan.ns;
+    if (int rc = check_tt_bits(bits, ns, true)) return rc;
+    const int r = K < ns ? e->plan.steps[size_t(K)].r : 0;
+    const int64_t n = int64_t(1) << r;
+    double L = -1e300;
+    for (int s = 0; s < nslices; s++) L = std::max(L, logscales[s]);
+    std::vector<unsigned char> raw(size_t(n) * e->esz);
+    double norm = 0;
+    for (int64_t i = 0; i < n; i++) {
+        std::complex<double> acc = 0;
+        for (int s = 0; s < nslices; s++)
+            acc += std::complex<double>(tables[s][2 * i], tables[s][2 * i + 1]) * std::exp2(logscales[s] - L);
+        if (e->storage == GS_C64) {
+            const float v[2] = {float(acc.real()), float(acc.imag())};
+            memcpy(raw.data() + size_t(i) * 8, v, 8);
+        } else {
+            const double v[2] = {acc.real(), acc.imag()};
+            memcpy(raw.data() + size_t(i) * 16, v, 16);
+        }
+        norm += std::norm(acc);
+    }
+    const double lp_prefix = -K + std::log2(norm) + 2 * L;
+    TT_TRY(hipSetDevice(e->device));
+    memcpy(e->h_forced, bits, size_t(ns));
+    std::mt19937_64 rng(seed + (unsigned long long)line);
+    std::uniform_real_distribution<double> U(0, 1);
+    int q = 0;
+    for (int k = K; k < ns; k++)
+        if (bits[k] < 0) e->h_U[q++] = U(rng);
+    e->cur = 0;
+    TT_TRY(hipMemcpyAsync(e->tab[0], raw.data(), raw.size(), hipMemcpyHostToDevice, e->stream));
+    TT_TRY(hipMemcpyAsync(e->d_forced, e->h_forced, size_t(ns), hipMemcpyHostToDevice, e->stream));
+    TT_TRY(hipMemcpyAsync(e->d_U, e->h_U, sizeof(double) * size_t(std::max(q, 1)), hipMemcpyHostToDevice, e->stream));
+    hipLaunchKernelGGL(tt_init_ctl_kernel, dim3(1), dim3(1), 0, e->stream, e->ctl, norm, L);
+    TT_TRY(hipGetLastError());
+    if (int rc = tt_enqueue(e, kRun, K, ns, e->h_forced, e->d_forced, e->d_U, e->d_bits, e->d_cond, e->d_out, nullptr))
+        return rc;
+    TT_TRY(hipMemcpyAsync(e->h_bits, e->d_bits, size_t(ns), hipMemcpyDeviceToHost, e->stream));
+    TT_TRY(hipMemcpyAsync(e->h_cond, e->d_cond, size_t(ns) * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    TT_TRY(hipMemcpyAsync(e->h_out, e->d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    TT_TRY(hipStreamSynchronize(e->stream));
+    for (int k = 0; k < ns; k++) out_bits[k] = k < K ? bits[k] : e->h_bits[k];
+    for (int k = K; k < ns; k++) cond[k] = e->h_cond[k];
+    *log2p_prefix = lp_prefix;
+    *log2p_full = lp_prefix + e->h_out[0];
+    return 0;
+}
+
 // dp.cpp:33-62: for each job, sum over its slices of pre * Z(w_slice), Z(w) = the final entry after every step with
 // the per-vertex weights w_v(0), w_v(1) (slice weights: [m][4] as w0re w0im w1re w1im, vertex-indexed).
 int gs_tt_dp(gs_tt_engine* e, int64_t njobs, const int* nslices, const double* pre, const double* w, double* out) {

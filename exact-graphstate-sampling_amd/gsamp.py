@@ -25,7 +25,7 @@ ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy
                "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms", "gs_tt_plan_load", "gs_tt_plan_info",
                "gs_tt_plan_destroy", "gs_tt_engine_create", "gs_tt_engine_destroy", "gs_tt_prob", "gs_tt_chain",
-               "gs_tt_sample", "gs_tt_hybrid", "gs_tt_dp", "gs_tt_last_error")
+               "gs_tt_sample", "gs_tt_hybrid", "gs_tt_prefix", "gs_tt_resume", "gs_tt_dp", "gs_tt_last_error")
 
 _lib = None
 
@@ -96,6 +96,8 @@ def library():
         "gs_tt_chain": (I, [P, i8p, I64, dp, dp, dp]),
         "gs_tt_sample": (I, [P, U64, I64, I64, i8p, dp]),
         "gs_tt_hybrid": (I, [P, i8p, I64, U64, I64, i8p, dp, dp]),
+        "gs_tt_prefix": (I, [P, i8p, I, dp, I64, ip, dp, dp]),
+        "gs_tt_resume": (I, [P, i8p, I, I, ctypes.POINTER(dp), dp, U64, I64, i8p, dp, dp, dp]),
         "gs_tt_dp": (I, [P, I64, ip, dp, dp, dp]),
         "gs_tt_last_error": (ctypes.c_char_p, []),
         "gs_batch_last_ms": (D, [P, dp]),
@@ -486,6 +488,32 @@ class TwoTableEngine:
         self._check(library().gs_tt_hybrid(self._h, _i8(b), b.shape[0], ctypes.c_uint64(seed), first_line, _i8(ob),
                                            _f8(lp), _f8(cond)))
         return ob, lp, cond
+
+    def prefix(self, bits, K):
+        """``(table [2^r] complex, r, logscale, log2 Pr(prefix))`` after the forced steps 0..K-1 (``gsamp prefix``)."""
+        b = np.ascontiguousarray(np.asarray(bits, dtype=np.int8).reshape(-1))
+        r = ctypes.c_int()
+        lib = library()
+        self._check(lib.gs_tt_prefix(self._h, _i8(b), K, None, 0, ctypes.byref(r), None, None))
+        n = 1 << r.value
+        tab = np.empty(2 * n)
+        ls, lp = ctypes.c_double(), ctypes.c_double()
+        self._check(lib.gs_tt_prefix(self._h, _i8(b), K, _f8(tab), n, ctypes.byref(r), ctypes.byref(ls),
+                                     ctypes.byref(lp)))
+        return tab[0::2] + 1j * tab[1::2], r.value, ls.value, lp.value
+
+    def resume(self, bits, K, tables, logscales, seed, line=0):
+        """``(bits, log2 Pr(full), log2 Pr(prefix), cond)`` (``gsamp resume``; -1 entries are sampled)."""
+        b = np.ascontiguousarray(np.asarray(bits, dtype=np.int8).reshape(-1))
+        tabs = [np.ascontiguousarray(np.stack([np.real(t), np.imag(t)], axis=1).reshape(-1)) for t in tables]
+        ptrs = (ctypes.POINTER(ctypes.c_double) * len(tabs))(*[_f8(t) for t in tabs])
+        ls = np.ascontiguousarray(np.asarray(logscales, dtype=np.float64))
+        ob = np.empty(self.ns, np.int8)
+        lpf, lpp = ctypes.c_double(), ctypes.c_double()
+        cond = np.zeros(self.ns)
+        self._check(library().gs_tt_resume(self._h, _i8(b), K, len(tabs), ptrs, _f8(ls), ctypes.c_uint64(seed), line,
+                                           _i8(ob), ctypes.byref(lpf), ctypes.byref(lpp), _f8(cond)))
+        return ob, lpf.value, lpp.value, cond
 
     def dp(self, jobs):
         """jobs: [[(pre complex, [(w0, w1)] per vertex)] per job] -> complex [njobs] (``dp``)."""

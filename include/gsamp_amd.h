@@ -129,6 +129,15 @@ int gs_tt_chain(gs_tt_engine* eng, const int8_t* bits, int64_t nstr, double* log
 int gs_tt_sample(gs_tt_engine* eng, uint64_t seed, int64_t first, int64_t count, int8_t* bits, double* log2p);
 int gs_tt_hybrid(gs_tt_engine* eng, const int8_t* bits, int64_t nstr, uint64_t seed, int64_t first_line,
                  int8_t* out_bits, double* log2p, double* cond);
+/* prefix (gsamp.cpp:124-135): forced steps 0..K-1, then the contiguous table of step K (2^r complex doubles in
+ * table_reim, capacity table_cap entries; r = st[K].r, 0 when K = ns) with its logscale and log2 Pr(prefix);
+ * table_reim = NULL only sets *r.  resume (gsamp.cpp:136-148): the nslices tables summed with weights
+ * 2^(ls_s - max ls), then steps K..ns-1 with '?' sampled from mt19937_64(seed + line). */
+int gs_tt_prefix(gs_tt_engine* eng, const int8_t* bits, int K, double* table_reim, int64_t table_cap, int* r,
+                 double* logscale, double* log2p_prefix);
+int gs_tt_resume(gs_tt_engine* eng, const int8_t* bits, int K, int nslices, const double* const* tables,
+                 const double* logscales, uint64_t seed, int64_t line, int8_t* out_bits, double* log2p_full,
+                 double* log2p_prefix, double* cond);
 /* dp (dp.cpp:33-62): out[j] = sum over job j's nslices[j] slices of pre * Z(w_slice); pre [total slices][2],
  * w [total slices][m][4] (w0re w0im w1re w1im per vertex), out [njobs][2]. */
 int gs_tt_dp(gs_tt_engine* eng, int64_t njobs, const int* nslices, const double* pre, const double* w, double* out);

@@ -7,6 +7,9 @@ gsamp.cpp:21-29  gather: output cp of the new table sums the inputs old[c], u = 
 gsamp.cpp:39-55  chain (prob mode, forced bits): per step both branch norms, p1 = n1 / (n0 + n1), log2 p_b, write the
                  chosen branch divided by sqrt(n_b).
 gsamp.cpp:59-70  chain1 (chain mode): one pass per step, cond = nb / (2 prev), log2 Pr = log2(prev) - ns.
+gsamp.cpp:73-91  run_steps (forced bits): per step scale by 1/sqrt(prev), cond = nb / 2, logscale += log2(prev) / 2.
+gsamp.cpp:124-148 prefix / resume: the table read by step K with its logscale; resume sums slice tables with weights
+                 2^(ls - max ls) and continues from prev = norm, logscale = max ls.
 dp.cpp:26-62     dp: per slice, every step with per-vertex weights (W0, W1); the job sums pre * table[0] over slices.
 
 Arithmetic is complex128 (numpy); ``store`` rounds each written table to complex64 as the reference's cf tables do
@@ -114,6 +117,44 @@ def chain1(steps, bits, c64=True):
         if nb <= 0:
             return -np.inf, cond
     return np.log2(prev) - len(steps), cond
+
+
+def run_steps(steps, bits, k0, k1, old, prev, logscale, c64=True):
+    """Forced steps k0..k1-1 from table ``old`` of norm ``prev`` (gsamp.cpp:73-91):
+    (table, prev, logscale, log2 Pr of the steps, cond)."""
+    logp = 0.0
+    cond = np.zeros(len(steps))
+    for k in range(k0, k1):
+        s, b = steps[k], int(bits[k])
+        sc = 1 / np.sqrt(prev)
+        S0, S1 = gather(s, old)
+        o = (s["M"][b, 0] * sc) * S0 + (s["M"][b, 1] * sc) * S1
+        nb = float(np.sum(np.abs(o) ** 2))
+        old = _store(o, c64)
+        logscale += 0.5 * np.log2(prev)
+        cond[k] = nb / 2
+        prev = nb
+        if nb <= 0:
+            return old, prev, logscale, -np.inf, cond
+        logp += np.log2(cond[k])
+    return old, prev, logscale, logp, cond
+
+
+def prefix(steps, bits, K, c64=True):
+    """(table read by step K, logscale, log2 Pr(prefix)) after forced steps 0..K-1 (gsamp.cpp:124-135)."""
+    tab, _, ls, lp, _ = run_steps(steps, bits, 0, K, np.ones(1, complex), 1.0, 0.0, c64)
+    return tab.astype(complex), ls, lp
+
+
+def resume(steps, bits, K, tables, logscales, c64=True):
+    """(log2 Pr(full), log2 Pr(prefix), cond) of the forced string continued from the summed slice tables
+    (gsamp.cpp:136-148)."""
+    L = max(logscales)
+    acc = sum(np.asarray(t, complex) * 2.0 ** (ls - L) for t, ls in zip(tables, logscales))
+    norm = float(np.sum(np.abs(acc) ** 2))
+    lpp = -K + np.log2(norm) + 2 * L
+    _, _, _, lps, cond = run_steps(steps, bits, K, len(steps), _store(acc, c64), norm, L, c64)
+    return lpp + lps, lpp, cond
 
 
 def dp(steps, jobs, c64=False):

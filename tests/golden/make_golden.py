@@ -179,6 +179,13 @@ def twotable():
                        ("hybrid", ["hybrid", hybrid, "99"])):
         with open(os.path.join(d, f"c1_{mode}.txt"), "w") as f:
             f.write(run_bin("gsamp", [plan] + args))
+    # prefix / resume (gsamp.cpp:124-148) of the hybrid lines at K = 18: dumps and lines
+    meta = json.load(open(os.path.join(HERE, "c1", "meta.json")))
+    K = meta["K"]
+    with open(os.path.join(d, "c1_prefix.txt"), "w") as f:
+        f.write(run_bin("gsamp", [plan, "prefix", hybrid, str(K), os.path.join(d, "c1_dump")]))
+    with open(os.path.join(d, "c1_resume.txt"), "w") as f:
+        f.write(run_bin("gsamp", [plan, "resume", hybrid, str(K), str(meta["resume_seed"]), os.path.join(d, "c1_dump")]))
     # dp: the graph with vertices 3 and 11 sliced, weights of 6 of the strings (w_q = M_q[b_q, :])
     A, M = inst.am.A, inst.am.M
     sl = egs.engine.Sliced(A, inst.order, [3, 11])

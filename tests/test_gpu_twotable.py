@@ -95,3 +95,62 @@ def test_cli_front_ends(tdir, golden, tmp_path):
                           os.path.join(tdir, "dp_jobs.txt")], capture_output=True, text=True, check=True).stdout.split()
     want = [float(x) for x in open(os.path.join(tdir, "dp_out_c128.txt")).read().split()]
     assert np.max(np.abs(np.array([float(x) for x in out]) - want)) < 1e-12 * np.max(np.abs(want))
+
+
+def test_c1_prefix_resume_vs_reference_gsamp(tdir, golden):
+    """gsamp prefix / resume at K = 18 on the hybrid lines: the dumped tables (the reference's own dump files) and the
+    resumed samples."""
+    import json
+    meta = json.load(open(os.path.join(golden, "c1", "meta.json")))
+    K = meta["K"]
+    ref_pre = lines(os.path.join(tdir, "c1_prefix.txt"))
+    ref_res = lines(os.path.join(tdir, "c1_resume.txt"))
+    print("two-table prefix/resume", flush=True)
+    with gs.TwoTableEngine(os.path.join(tdir, "c1_plan.txt")) as e:
+        hy = egs.planio.read_strings(os.path.join(golden, "c1", "hybrid.txt"), e.ns, allow_q=True)
+        assert len(hy) == len(ref_pre) == len(ref_res)
+        for li, row in enumerate(hy):
+            tab, r, ls, lp = e.prefix(row, K)
+            rtab, rr, rone, rls = egs.planio.read_dump(os.path.join(tdir, f"c1_dump_{li}.bin"))
+            assert (r, 1) == (rr, rone)
+            assert abs(ls - rls) < 1e-6 and abs(lp - float(ref_pre[li][1])) < 1e-5
+            assert np.max(np.abs(tab - rtab)) < 1e-5 * np.max(np.abs(rtab))
+            bits, lpf, lpp, cond = e.resume(row, K, [rtab], [rls], meta["resume_seed"], li)
+            assert egs.planio.format_bits(bits) == ref_res[li][0]
+            assert abs(lpf - float(ref_res[li][1])) < 1e-5 and abs(lpp - float(ref_res[li][2])) < 1e-5
+            assert np.max(np.abs(cond[K:] - [float(x) for x in ref_res[li][3:]])) < 1e-6
+
+
+def test_prefix_resume_cli_and_two_slices(tdir, golden, tmp_path):
+    """gsamp_amd prefix writes the reference's dump files; resume over two dump prefixes (the table split into two
+    weighted slices) reproduces the reference's single-slice resume lines."""
+    import json
+    meta = json.load(open(os.path.join(golden, "c1", "meta.json")))
+    K = meta["K"]
+    exe = os.path.join(ROOT, "exact-graphstate-sampling_amd", "bin", "gsamp_amd")
+    plan = os.path.join(tdir, "c1_plan.txt")
+    hyf = os.path.join(golden, "c1", "hybrid.txt")
+    out = subprocess.run([exe, plan, "prefix", hyf, str(K), str(tmp_path / "d")], capture_output=True, text=True,
+                         check=True).stdout.splitlines()
+    ref_pre = lines(os.path.join(tdir, "c1_prefix.txt"))
+    assert len(out) == len(ref_pre)
+    for li, l in enumerate(out):
+        assert l.split()[0] == ref_pre[li][0] and abs(float(l.split()[1]) - float(ref_pre[li][1])) < 1e-5
+        got, r, one, ls = egs.planio.read_dump(str(tmp_path / f"d_{li}.bin"))
+        want, rr, _, rls = egs.planio.read_dump(os.path.join(tdir, f"c1_dump_{li}.bin"))
+        assert (r, one) == (rr, 1) and abs(ls - rls) < 1e-6
+        assert np.max(np.abs(got - want)) < 1e-5 * np.max(np.abs(want))
+        # split the reference's table into two slices: a * 2^-2 at logscale ls + 2 and the rest at ls - 1
+        a = want * np.linspace(0.25, 0.75, want.size)
+        for tag, t, s in (("a", a * 0.25, rls + 2), ("b", (want - a) * 2.0, rls - 1)):
+            with open(tmp_path / f"{tag}_{li}.bin", "wb") as f:
+                f.write(np.array([r, 1], np.int32).tobytes() + np.array([s]).tobytes())
+                f.write(np.stack([t.real, t.imag], 1).astype(np.float32).tobytes())
+    out = subprocess.run([exe, plan, "resume", hyf, str(K), str(meta["resume_seed"]), str(tmp_path / "a"),
+                          str(tmp_path / "b")], capture_output=True, text=True, check=True).stdout.splitlines()
+    ref_res = lines(os.path.join(tdir, "c1_resume.txt"))
+    assert len(out) == len(ref_res)
+    for l, want in zip(out, ref_res):
+        got = l.split()
+        assert got[0] == want[0]
+        assert np.max(np.abs(np.array([float(x) for x in got[1:]]) - [float(x) for x in want[1:]])) < 1e-5

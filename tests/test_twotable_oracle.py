@@ -57,3 +57,44 @@ def test_dp_matches_reference_and_in_place_amplitude(tdir, golden):
     ns = 24
     ref_ip = np.array([float(l[0]) for l in lines(os.path.join(golden, "c1", "prob_c128.txt"))][:6])
     assert np.max(np.abs(np.log2(np.abs(want) ** 2) - ns - ref_ip)) < 1e-9
+
+
+def test_prefix_and_resume_match_reference_gsamp(tdir, golden):
+    """prefix at K = 18 against the reference's dump files and printed lines; resume with the bits the reference drew
+    (its '?' outcomes taken from its output) against its printed probabilities and conditionals."""
+    import json
+    meta = json.load(open(os.path.join(golden, "c1", "meta.json")))
+    K = meta["K"]
+    _, _, steps = tt.load_plan(os.path.join(tdir, "c1_plan.txt"))
+    ref_pre = lines(os.path.join(tdir, "c1_prefix.txt"))
+    ref_res = lines(os.path.join(tdir, "c1_resume.txt"))
+    hy = egs.planio.read_strings(os.path.join(golden, "c1", "hybrid.txt"), len(steps), allow_q=True)
+    assert len(hy) == len(ref_pre) == len(ref_res) == 8
+    for li, row in enumerate(hy):
+        tab, ls, lp = tt.prefix(steps, row, K)
+        rtab, r, one, rls = egs.planio.read_dump(os.path.join(tdir, f"c1_dump_{li}.bin"))
+        assert (one, len(tab)) == (1, 1 << r) and r == steps[K]["r"]
+        assert abs(ls - rls) < 1e-6 and abs(lp - float(ref_pre[li][1])) < 1e-9
+        assert np.max(np.abs(tab - rtab)) < 1e-6 * np.max(np.abs(rtab))
+        bits = np.array([int(c) for c in ref_res[li][0]])
+        assert all(bits[k] == row[k] for k in range(len(steps)) if row[k] >= 0)
+        lpf, lpp, cond = tt.resume(steps, bits, K, [rtab], [rls])
+        assert abs(lpf - float(ref_res[li][1])) < 1e-9 and abs(lpp - float(ref_res[li][2])) < 1e-9
+        assert np.max(np.abs(cond[K:] - [float(x) for x in ref_res[li][3:]])) < 2e-9
+
+
+def test_resume_of_two_slices_is_linear(tdir, golden):
+    """Resume of a table split into two weighted slices equals resume of the whole table (the slice sum is linear;
+    logscales shift the weights)."""
+    import json
+    meta = json.load(open(os.path.join(golden, "c1", "meta.json")))
+    K = meta["K"]
+    _, _, steps = tt.load_plan(os.path.join(tdir, "c1_plan.txt"))
+    ref_res = lines(os.path.join(tdir, "c1_resume.txt"))
+    rtab, _, _, rls = egs.planio.read_dump(os.path.join(tdir, "c1_dump_0.bin"))
+    bits = np.array([int(c) for c in ref_res[0][0]])
+    rng = np.random.default_rng(3)
+    part = rtab * rng.uniform(0.2, 0.8, rtab.size)
+    whole = tt.resume(steps, bits, K, [rtab], [rls], c64=False)
+    split = tt.resume(steps, bits, K, [part * 2.0 ** -3, (rtab - part) * 2.0 ** 1], [rls + 3, rls - 1], c64=False)
+    assert abs(whole[0] - split[0]) < 1e-9 and np.max(np.abs(whole[2] - split[2])) < 1e-9
