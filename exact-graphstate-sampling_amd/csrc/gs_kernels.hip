@@ -28,11 +28,17 @@
 // Tiles are enumerated in Gray-code order inside a block, so the next tile XORs one column into the base address and
 // flips the parity bits by one mask: O(1) scalar work per tile.
 //
-// Chain logic.  The last block of every sweep (detected with an agent-scope atomic counter after each block has
-// published its partial sums) reduces the partials in a fixed order and runs the chain on the device: the bookkeeping
-// of a write (chain1 / run_steps conditionals, the stored table's norm) and the decision of the next step (forced, or
-// U_k < p1 with U_k drawn on the host: gsamp2.cpp:47, :86, :108 draw one uniform per sampled step from a stream that
-// does not depend on p1).  No host round trip, no separate reduction launch.
+// Chain logic.  The block partial sums of a sweep are reduced in a fixed order and the chain runs on the device: the
+// bookkeeping of a write (chain1 / run_steps conditionals, the stored table's norm) and the decision of the next step
+// (forced, or U_k < p1 with U_k drawn on the host: gsamp2.cpp:47, :86, :108 draw one uniform per sampled step from a
+// stream that does not depend on p1).  Sweeps of a few blocks reduce in their last block (agent-scope counter);
+// larger ones are followed by a one-block reduction kernel.  No host round trip.
+//
+// Arithmetic per coset.  The sign of an x = 1 term is (-1)^{pc} sigma_j with pc = parity(c & rhoc) (per coset) and
+// sigma_j = (-1)^{par_j} (per step), so A[tau][1] = (-1)^{pc} sigma_{j0} B_tau with B_tau = sum_j sigma_{j0} sigma_j
+// slot_j: sigma_{j0} goes into the step's coefficient and (-1)^{pc} flips the fresh product M_b1 B_tau (one XOR per
+// component).  The fused sweeps reduce the next step's Gram matrix from per-route invariants of each next-step coset
+// (a, b): |a|^2 + |b|^2, Re/Im a conj(b), |a|^2 - |b|^2, with the per-coset signs applied to scalars.
 #include "gs_device.h"
 
 namespace gs {
@@ -176,6 +182,8 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ double norm2(double2 a) { return fma(a.x, a.x, a.y * a.y); }
+// acc += |a|^2 as two fused multiply-adds
+__device__ __forceinline__ void add_norm(double& acc, double2 a) { acc = fma(a.x, a.x, fma(a.y, a.y, acc)); }
 
 // output index of offset j's x = 0 / x = 1 term under each canonical route
 template <int ROUTE>
@@ -188,10 +196,12 @@ __device__ __host__ constexpr int tau_x1(int j) {
 }
 
 // Accumulate the routed signed slot sums of one coset: A[tau][x] += slot values (sign (-1)^{pc ^ par_j} on x = 1).
+// The sums start from -0.0, the additive identity (x + -0.0 == x for every x), so a sum of one term compiles to no
+// addition at all.
 template <int ROUTE, int NTQ, int MO>
 __device__ __forceinline__ void route_terms(double2 (&A)[MO][2], const double2 (&w)[NTQ], int pc, uint32_t parmask) {
 #pragma unroll
-    for (int t = 0; t < MO; t++) A[t][0] = A[t][1] = make_double2(0, 0);
+    for (int t = 0; t < MO; t++) A[t][0] = A[t][1] = make_double2(-0.0, -0.0);
 #pragma unroll
     for (int j = 0; j < NTQ; j++) {
         A[tau_x0<ROUTE>(j)][0] = cadd(A[tau_x0<ROUTE>(j)][0], w[j]);
@@ -204,8 +214,8 @@ __device__ __forceinline__ void gram_acc(const double2 (&A)[MO][2], double* acc)
 #pragma unroll
     for (int t = 0; t < MO; t++) {
         const double2 a = A[t][0], b = A[t][1];
-        acc[0] += norm2(a);
-        acc[1] += norm2(b);
+        add_norm(acc[0], a);
+        add_norm(acc[1], b);
         acc[2] = fma(a.x, b.x, fma(a.y, b.y, acc[2]));      // Re a conj(b)
         acc[3] = fma(a.y, b.x, fma(-a.x, b.y, acc[3]));     // Im a conj(b)
     }
@@ -215,8 +225,110 @@ template <int MO>
 __device__ __forceinline__ void direct_acc(const double2 (&A)[MO][2], const double2 (&M)[2][2], double* acc) {
 #pragma unroll
     for (int t = 0; t < MO; t++) {
-        acc[0] += norm2(cfma(M[0][0], A[t][0], cmul(M[0][1], A[t][1])));
-        acc[1] += norm2(cfma(M[1][0], A[t][0], cmul(M[1][1], A[t][1])));
+        add_norm(acc[0], cfma(M[0][0], A[t][0], cmul(M[0][1], A[t][1])));
+        add_norm(acc[1], cfma(M[1][0], A[t][0], cmul(M[1][1], A[t][1])));
+    }
+}
+
+// (-1)^s v for s in {0, 1} (sign bit flip)
+__device__ __forceinline__ double flip(double v, uint32_t s) {
+    return __hiloint2double(__double2hiint(v) ^ int(s << 31), __double2loint(v));
+}
+__device__ __forceinline__ double2 flip2(double2 v, uint32_t s) { return make_double2(flip(v.x, s), flip(v.y, s)); }
+
+// First offset routed to (tau, x = 1), or -1.
+template <int ROUTE, int NTQ>
+__device__ __host__ constexpr int first_x1(int tau) {
+    for (int j = 0; j < NTQ; j++)
+        if (tau_x1<ROUTE>(j) == tau) return j;
+    return -1;
+}
+
+// Per-launch write coefficients: w1t[tau] = sigma_{j0(tau)} w1 and rho[j] = sigma_{j0} sigma_j (j0 = first_x1 of
+// offset j's x = 1 output).
+template <int ROUTE, int NTQ, int MO>
+__device__ __forceinline__ void write_coefs(double2 w1, uint32_t parmask, double2 (&w1t)[MO], double (&rho)[NTQ]) {
+#pragma unroll
+    for (int t = 0; t < MO; t++) {
+        const int j0 = first_x1<ROUTE, NTQ>(t);
+        w1t[t] = j0 >= 0 ? flip2(w1, (parmask >> j0) & 1) : w1;
+    }
+#pragma unroll
+    for (int j = 0; j < NTQ; j++) {
+        const int j0 = first_x1<ROUTE, NTQ>(tau_x1<ROUTE>(j));
+        rho[j] = flip(1.0, ((parmask >> j) ^ (parmask >> j0)) & 1);
+    }
+}
+
+// Output tau of one coset: w0 A[tau][0] + (-1)^pc w1t[tau] B_tau (the slot sums in offset order, as route_terms).
+template <int ROUTE, int NTQ, int MO>
+__device__ __forceinline__ double2 write_out(int t, const double2 (&x)[NTQ], uint32_t pc, double2 w0,
+                                             const double2 (&w1t)[MO], const double (&rho)[NTQ]) {
+    double2 a = make_double2(0, 0), b = a;
+    bool h0 = false, h1 = false;
+#pragma unroll
+    for (int j = 0; j < NTQ; j++) {
+        if (tau_x0<ROUTE>(j) == t) {
+            a = h0 ? cadd(a, x[j]) : x[j];
+            h0 = true;
+        }
+        if (tau_x1<ROUTE>(j) == t) {
+            b = h1 ? make_double2(fma(rho[j], x[j].x, b.x), fma(rho[j], x[j].y, b.y)) : x[j];
+            h1 = true;
+        }
+    }
+    if (!h1) return cmul(w0, a);
+    const double2 p = flip2(cmul(w1t[t], b), pc);
+    return h0 ? cfma(w0, a, p) : p;
+}
+
+// Next-step Gram invariants of one coset (va, vb) in coset order; J swaps the roles of the two slots, sg is the
+// coset's sign bit.  Accumulators: N = sum |va|^2 + |vb|^2, R = sum Re va conj(vb), SR = sum (-1)^sg Re, SI = sum
+// (-1)^{sg ^ J} Im, SN = sum (-1)^sg N, SD = sum (-1)^{sg ^ J} (|va|^2 - |vb|^2).
+enum { kgN, kgR, kgSR, kgSI, kgSN, kgSD, kgCount };
+template <int ROUTE2>
+__device__ __forceinline__ void gram_invariants(double2 va, double2 vb, uint32_t J, uint32_t sg, double (&g)[kgCount]) {
+    if constexpr (ROUTE2 == kRouteSplit || ROUTE2 == kRouteMixA) {
+        add_norm(g[kgN], va);
+        add_norm(g[kgN], vb);
+    } else {
+        const double na = norm2(va), nb = norm2(vb);
+        g[kgN] += na + nb;
+        g[kgSN] += flip(na + nb, sg);
+        g[kgSD] += flip(na - nb, sg ^ J);
+    }
+    if constexpr (ROUTE2 == kRouteSplit) {
+        g[kgR] = fma(va.x, vb.x, fma(va.y, vb.y, g[kgR]));
+    } else if constexpr (ROUTE2 == kRouteMixA || ROUTE2 == kRoute1) {
+        const double re = fma(va.x, vb.x, va.y * vb.y), im = fma(va.y, vb.x, -va.x * vb.y);
+        if constexpr (ROUTE2 == kRoute1) g[kgR] += re;
+        g[kgSR] += flip(re, sg);
+        g[kgSI] += flip(im, sg ^ J);
+    }
+}
+
+// G00, G11, Re G01, Im G01 of the next step from the invariants (sigma1 = (-1)^{par} of its second slot).
+template <int ROUTE2>
+__device__ __forceinline__ void gram_assemble(const double (&g)[kgCount], int par, double* G) {
+    const double s1 = par ? -1.0 : 1.0;
+    if constexpr (ROUTE2 == kRouteSplit) {
+        G[0] = g[kgN] + 2 * g[kgR];
+        G[1] = g[kgN] + 2 * s1 * g[kgR];
+        G[2] = 0;
+        G[3] = 0;
+    } else if constexpr (ROUTE2 == kRouteMixA) {
+        G[0] = G[1] = g[kgN];
+        G[2] = (1 + s1) * g[kgSR];
+        G[3] = (s1 - 1) * g[kgSI];
+    } else if constexpr (ROUTE2 == kRouteMixB) {
+        G[0] = G[1] = g[kgN];
+        G[2] = par ? g[kgSD] : g[kgSN];
+        G[3] = 0;
+    } else {
+        G[0] = g[kgN] + 2 * g[kgR];
+        G[1] = g[kgN] + 2 * s1 * g[kgR];
+        G[2] = (par ? g[kgSD] : g[kgSN]) + (1 + s1) * g[kgSR];
+        G[3] = (s1 - 1) * g[kgSI];
     }
 }
 
@@ -481,9 +593,12 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
 
     double2 w0 = make_double2(0, 0), w1 = w0;
     double2 M[2][2] = {{w0, w0}, {w0, w0}};
+    double2 w1t[MO];
+    double rho[NTQ];
     if constexpr (WRITES) {
         w0 = make_double2(ctl->w[0], ctl->w[1]);
         w1 = make_double2(ctl->w[2], ctl->w[3]);
+        write_coefs<ROUTE, NTQ, MO>(w1, s.parmask, w1t, rho);
     }
     if constexpr (SWEEP == kDirect || SWEEP == kWriteDirect) {
         const KStep& ms = SWEEP == kDirect ? epi.plain[epi.k] : epi.plain[epi.k + 1];
@@ -503,7 +618,9 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     double acc[NA];
 #pragma unroll
     for (int q = 0; q < NA; q++) acc[q] = 0;
-    const bool want_nb = epi.mode != kChain;
+    double gi[kgCount];                            // next-step Gram invariants (kWriteGram)
+#pragma unroll
+    for (int q = 0; q < kgCount; q++) gi[q] = 0;
 
     // Software pipeline: the loads of tile it + 1 are issued before tile it is computed and stored.  Tiles hold
     // disjoint cosets, and a coset's outputs go to its own slots (or to fresh slots outside the live space for growth
@@ -553,40 +670,39 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
                 double2 wv[NTQ];
 #pragma unroll
                 for (int j = 0; j < NTQ; j++) wv[j] = Cx<S>::widen(e[j][ig][ir]);
-                double2 A[MO][2];
-                route_terms<ROUTE, NTQ, MO>(A, wv, pc, parmask);
-                if constexpr (SWEEP == kGram) {
-                    gram_acc<MO>(A, acc);
-                } else if constexpr (SWEEP == kDirect) {
-                    direct_acc<MO>(A, M, acc);
+                if constexpr (SWEEP == kGram || SWEEP == kDirect) {
+                    double2 A[MO][2];
+                    route_terms<ROUTE, NTQ, MO>(A, wv, pc, parmask);
+                    if constexpr (SWEEP == kGram) gram_acc<MO>(A, acc);
+                    else direct_acc<MO>(A, M, acc);
                 } else {
 #pragma unroll
                     for (int t = 0; t < MO; t++) {
-                        const double2 v = cfma(w0, A[t][0], cmul(w1, A[t][1]));
+                        const double2 v = write_out<ROUTE, NTQ, MO>(t, wv, uint32_t(pc), w0, w1t, rho);
                         const S sv = Cx<S>::narrow(v);
                         o[t][ig][ir] = sv;
                         // the written norm feeds chain1 / run_steps bookkeeping only (sampling renormalises by the
-                        // decided branch's norm); the stored table's norm is read only by a growth step's analytic
-                        // norms, and a fused write is never followed by a growth step
-                        if (want_nb) acc[0] += norm2(v);
-                        if constexpr (!FUSED) acc[1] += norm2(Cx<S>::widen(sv));
+                        // decided branch's norm; accumulating it always is cheaper than selecting); the stored
+                        // table's norm is read only by a growth step's analytic norms, and a fused write is never
+                        // followed by a growth step
+                        add_norm(acc[0], v);
+                        if constexpr (!FUSED) add_norm(acc[1], Cx<S>::widen(sv));
                     }
                 }
             }
         if constexpr (WRITES) {
+            // stores go through a permuted copy; o stays in coset order for the fused norms
 #pragma unroll
-            for (int ig = 0; ig < VG; ig++) {
-                const uint64_t a0 = ag ^ (ig ? gam : 0);
-                permute<S, VR>(o[0][ig], unsigned(a0) & (VR - 1));
-                Run<S, VR>::store(mem, a0 & ~uint64_t(VR - 1), o[0][ig]);
-                permute<S, VR>(o[0][ig], unsigned(a0) & (VR - 1));     // back to coset order for the fused norms
-                if constexpr (MO == 2) {
-                    const uint64_t a1 = a0 ^ g;
-                    permute<S, VR>(o[1][ig], unsigned(a1) & (VR - 1));
-                    Run<S, VR>::store(mem, a1 & ~uint64_t(VR - 1), o[1][ig]);
-                    permute<S, VR>(o[1][ig], unsigned(a1) & (VR - 1));
+            for (int ig = 0; ig < VG; ig++)
+#pragma unroll
+                for (int t = 0; t < MO; t++) {
+                    const uint64_t a = ag ^ (ig ? gam : 0) ^ (t ? g : 0);
+                    S so[VR];
+#pragma unroll
+                    for (int ir = 0; ir < VR; ir++) so[ir] = o[t][ig][ir];
+                    permute<S, VR>(so, unsigned(a) & (VR - 1));
+                    Run<S, VR>::store(mem, a & ~uint64_t(VR - 1), so);
                 }
-            }
         }
         if constexpr (FUSED) {
             // next step's cosets: output (ir, ig=0, tau d) pairs with output (ir, ig=1, tau d ^ eps); the element with
@@ -598,17 +714,20 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
                     const int pl = pbase ^ plow[ir];
                     const int jidx = ((pl >> 1) ^ (d & phig)) & 1;
                     const int sg = ((pl >> 2) ^ (d & nug)) & 1;
-                    const double2 va = Cx<S>::widen(o[d][0][ir]);
-                    double2 vb;
-                    if constexpr (MO == 2) vb = Cx<S>::widen((d ^ eps) ? o[1][1][ir] : o[0][1][ir]);
-                    else vb = Cx<S>::widen(o[0][1][ir]);
-                    double2 w2[2];
-                    w2[0] = jidx ? vb : va;
-                    w2[1] = jidx ? va : vb;
-                    double2 A2[MO2][2];
-                    route_terms<ROUTE2, 2, MO2>(A2, w2, sg, par2mask);
-                    if constexpr (SWEEP == kWriteGram) gram_acc<MO2>(A2, acc + 2);
-                    else direct_acc<MO2>(A2, M, acc + 2);
+                    const S va = o[d][0][ir];
+                    S vb;
+                    if constexpr (MO == 2) vb = (d ^ eps) ? o[1][1][ir] : o[0][1][ir];
+                    else vb = o[0][1][ir];
+                    if constexpr (SWEEP == kWriteGram) {
+                        gram_invariants<ROUTE2>(Cx<S>::widen(va), Cx<S>::widen(vb), uint32_t(jidx), uint32_t(sg), gi);
+                    } else {
+                        double2 w2[2];              // select in the stored type, then widen
+                        w2[0] = Cx<S>::widen(jidx ? vb : va);
+                        w2[1] = Cx<S>::widen(jidx ? va : vb);
+                        double2 A2[MO2][2];
+                        route_terms<ROUTE2, 2, MO2>(A2, w2, sg, par2mask);
+                        direct_acc<MO2>(A2, M, acc + 2);
+                    }
                 }
         }
 #pragma unroll
@@ -618,6 +737,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
 #pragma unroll
                 for (int ir = 0; ir < VR; ir++) e[j][ig][ir] = en[j][ig][ir];
     }
+    if constexpr (SWEEP == kWriteGram) gram_assemble<ROUTE2>(gi, s.par2, acc + 2);
     sweep_epilogue<NA>(acc, part, ctl, epi, SWEEP);
 }
 
@@ -666,8 +786,8 @@ __device__ void small_pass(const KStep& s, S* mem, int sweep, const double2 (&M)
                 const double2 v = cfma(w0, A[t][0], cmul(w1, A[t][1]));
                 const S sv = Cx<S>::narrow(v);
                 mem[t ? base ^ s.g : base] = sv;
-                acc[0] += norm2(v);
-                acc[1] += norm2(Cx<S>::widen(sv));
+                add_norm(acc[0], v);
+                add_norm(acc[1], Cx<S>::widen(sv));
             }
         }
     }
@@ -804,15 +924,15 @@ __global__ __launch_bounds__(kThreads) void gather_kernel(const TStep* __restric
             else S0 = cadd(S0, v);
         }
         if constexpr (SWEEP == kDirect) {
-            acc[0] += norm2(cfma(M[0][0], S0, cmul(M[0][1], S1)));
-            acc[1] += norm2(cfma(M[1][0], S0, cmul(M[1][1], S1)));
+            add_norm(acc[0], cfma(M[0][0], S0, cmul(M[0][1], S1)));
+            add_norm(acc[1], cfma(M[1][0], S0, cmul(M[1][1], S1)));
         } else {
             const double2 o = cfma(w0, S0, cmul(w1, S1));
             const S so = Cx<S>::narrow(o);
             const uint64_t cp = c_it | (uint64_t(blockIdx.x) << 8) | uint64_t(tid);
             Run<S, 1>::store(nw, cp, {so});
-            acc[0] += norm2(o);
-            acc[1] += norm2(Cx<S>::widen(so));
+            add_norm(acc[0], o);
+            add_norm(acc[1], Cx<S>::widen(so));
         }
     }
     sweep_epilogue<2>(acc, part, ctl, epi, SWEEP);
