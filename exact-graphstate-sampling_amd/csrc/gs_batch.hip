@@ -37,6 +37,7 @@ namespace {
 
 constexpr int kLanes = 64;           // strings per wave (one lane group)
 constexpr int kWaves = 4;            // waves per block, each on its own part of the coset range
+constexpr int kRChunks = 64;         // chunks of the first reduction stage per lane group
 constexpr int kBThreads = kLanes * kWaves;
 
 struct BStep {
@@ -70,6 +71,7 @@ struct BInst {           // device view of one instance
     double* amp;         // [B][2]
     double* grad;        // [B][ns][2][2]: G_k(x) re, im (gradient mode)
     double* part;        // [groups][max items per group][64][4]
+    double* cpart;       // [groups][kRChunks][64][4]: chunk sums of the item partials
     int B, ns, pad0, pad1;
 };
 
@@ -391,8 +393,39 @@ __global__ __launch_bounds__(kBThreads) void batch_back_kernel(const BInst* __re
     lane_store<4>(I, it, wave, lane, val);
 }
 
-// After step k: per (instance, lane), the fixed-order sum of the items' partials (cnt[inst] items per lane group),
-// then chain1 bookkeeping (mode 0: cond, prev, dead; gsamp2.cpp:67) or the gradient entry G_k (mode 1).
+// After step k, in two fixed-order stages: batch_chunk_kernel sums the items' partials (cnt[inst] items per lane
+// group) in kRChunks chunks of consecutive items, one block of kWaves waves per (lane group, chunk) -- wave w takes
+// items w, w + kWaves, ... of the chunk -- and batch_reduce_kernel sums the chunks per (instance, lane), then runs
+// chain1 bookkeeping (mode 0: cond, prev, dead; gsamp2.cpp:67) or stores the gradient entry G_k (mode 1).
+__host__ __device__ inline int chunk_items(int count) { return (count + kRChunks - 1) / kRChunks; }
+
+__global__ __launch_bounds__(kBThreads) void batch_chunk_kernel(const BInst* __restrict__ insts,
+                                                               const int* __restrict__ cnt, int mode) {
+    const int inst = blockIdx.y, group = blockIdx.x / kRChunks, chunk = blockIdx.x % kRChunks;
+    const int lane = threadIdx.x & (kLanes - 1), wave = threadIdx.x / kLanes;
+    const BInst& I = insts[inst];
+    const int count = cnt[inst];
+    if (count == 0 || group >= I.B / kLanes) return;
+    const int per = chunk_items(count);
+    const int i0 = chunk * per, i1 = min(count, i0 + per);
+    if (i0 >= i1) return;
+    const int nq = mode == 0 ? 1 : 4;
+    const double* gp = I.part + size_t(group) * size_t(count) * kLanes * 4;
+    double tot[4] = {0, 0, 0, 0};
+    for (int i = i0 + wave; i < i1; i += kWaves)
+        for (int q = 0; q < nq; q++) tot[q] += gp[(size_t(i) * kLanes + lane) * 4 + q];
+    __shared__ double sh[kWaves][kLanes][4];
+    for (int q = 0; q < 4; q++) sh[wave][lane][q] = tot[q];
+    __syncthreads();
+    if (wave != 0) return;
+    double* out = I.cpart + ((size_t(group) * kRChunks + chunk) * kLanes + lane) * 4;
+    for (int q = 0; q < nq; q++) {
+        double t = sh[0][lane][q];
+        for (int w = 1; w < kWaves; w++) t += sh[w][lane][q];
+        out[q] = t;
+    }
+}
+
 __global__ __launch_bounds__(kLanes) void batch_reduce_kernel(const BInst* __restrict__ insts,
                                                               const int* __restrict__ cnt, int k, int mode) {
     const int inst = blockIdx.y, group = blockIdx.x, lane = threadIdx.x;
@@ -402,9 +435,10 @@ __global__ __launch_bounds__(kLanes) void batch_reduce_kernel(const BInst* __res
     const int b = group * kLanes + lane;
     const int nq = mode == 0 ? 1 : 4;
     double tot[4] = {0, 0, 0, 0};
-    const double* gp = I.part + size_t(group) * size_t(count) * kLanes * 4;
-    for (int i = 0; i < count; i++)
-        for (int q = 0; q < nq; q++) tot[q] += gp[(size_t(i) * kLanes + lane) * 4 + q];
+    const int nchunks = (count + chunk_items(count) - 1) / chunk_items(count);
+    const double* gp = I.cpart + size_t(group) * kRChunks * kLanes * 4;
+    for (int c = 0; c < nchunks; c++)
+        for (int q = 0; q < nq; q++) tot[q] += gp[(size_t(c) * kLanes + lane) * 4 + q];
     const int ns = I.ns;
     if (mode == 1) {
         double* G = I.grad + (size_t(b) * ns + k) * 4;
@@ -625,6 +659,8 @@ int launch_batch(gs_batch* bt, BKernel kind, int k, BLaunch L) {
 // Per-(instance, lane) reduction of step k's item partials: mode 0 chain bookkeeping, mode 1 gradient entry.
 int launch_reduce(gs_batch* bt, int k, int mode) {
     if (bt->nitems[size_t(k)] == 0) return 0;
+    hipLaunchKernelGGL(batch_chunk_kernel, dim3(bt->maxgroups * kRChunks, int(bt->plans.size())), dim3(kBThreads), 0,
+                       bt->stream, bt->dinst, bt->dcnt + size_t(k) * bt->plans.size(), mode);
     hipLaunchKernelGGL(batch_reduce_kernel, dim3(bt->maxgroups, int(bt->plans.size())), dim3(kLanes), 0, bt->stream,
                        bt->dinst, bt->dcnt + size_t(k) * bt->plans.size(), k, mode);
     return hipGetLastError() == hipSuccess ? 0 : bfail(17, "reduce launch failed");
@@ -929,7 +965,8 @@ int gs_batch_create(const gs_plan* const* plans, const int* lanes, int nplans, i
         I.logp = dalloc<double>(bt, size_t(B));
         I.amp = dalloc<double>(bt, size_t(B) * 2);
         I.part = dalloc<double>(bt, size_t(B / kLanes) * size_t(max_items(bt, size_t(i))) * kLanes * 4);
-        if (!I.bitsT || !I.prev || !I.dead || !I.cond || !I.logp || !I.amp || !I.part)
+        I.cpart = dalloc<double>(bt, size_t(B / kLanes) * kRChunks * kLanes * 4);
+        if (!I.bitsT || !I.prev || !I.dead || !I.cond || !I.logp || !I.amp || !I.part || !I.cpart)
             return bail(bfail(11, "batch buffer allocation failed"));
         bt->hinst.push_back(I);
     }
