@@ -85,18 +85,21 @@ struct BLaunch {
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef double d2v __attribute__((ext_vector_type(2)));
 
-// Arena entries: raw (non-temporal) loads and stores of the storage type, widening to FP64 where used.
+// Arena entries: raw (non-temporal) loads and stores of the storage type, widening to FP64 where used.  The arena
+// pointers come from the instance table in memory, so the compiler cannot tell their address space; the accesses
+// name the global address space explicitly (global instead of flat instructions).
+#define GS_GLOBAL __attribute__((address_space(1)))
 template <class S>
 struct Sto;
 template <>
 struct Sto<float2> {
     static __device__ __forceinline__ float2 ld(const float2* p) {
-        const f2v t = __builtin_nontemporal_load(reinterpret_cast<const f2v*>(p));
+        const f2v t = __builtin_nontemporal_load((const GS_GLOBAL f2v*)(p));
         return make_float2(t.x, t.y);
     }
     static __device__ __forceinline__ void st(float2* p, float2 v) {
         const f2v t = {v.x, v.y};
-        __builtin_nontemporal_store(t, reinterpret_cast<f2v*>(p));
+        __builtin_nontemporal_store(t, (GS_GLOBAL f2v*)(p));
     }
     static __device__ __forceinline__ double2 wide(float2 v) { return make_double2(v.x, v.y); }
     static __device__ __forceinline__ float2 narrow(double2 v) { return make_float2(float(v.x), float(v.y)); }
@@ -104,12 +107,12 @@ struct Sto<float2> {
 template <>
 struct Sto<double2> {
     static __device__ __forceinline__ double2 ld(const double2* p) {
-        const d2v t = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+        const d2v t = __builtin_nontemporal_load((const GS_GLOBAL d2v*)(p));
         return make_double2(t.x, t.y);
     }
     static __device__ __forceinline__ void st(double2* p, double2 v) {
         const d2v t = {v.x, v.y};
-        __builtin_nontemporal_store(t, reinterpret_cast<d2v*>(p));
+        __builtin_nontemporal_store(t, (GS_GLOBAL d2v*)(p));
     }
     static __device__ __forceinline__ double2 wide(double2 v) { return v; }
     static __device__ __forceinline__ double2 narrow(double2 v) { return v; }
@@ -126,6 +129,14 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ double norm2(double2 a) { return fma(a.x, a.x, a.y * a.y); }
+
+// The step's coordinate columns in LDS, for the whole block (the Gray walk reads one per coset group; read through
+// the instance table in memory, each would be a dependent memory round trip).  Every thread of the block calls it.
+__device__ __forceinline__ const uint64_t* lds_cols(const uint64_t* __restrict__ cols, int nk, uint64_t* sc) {
+    for (int j = threadIdx.x; j < nk; j += blockDim.x) sc[j] = cols[j];
+    __syncthreads();
+    return sc;
+}
 
 // Coset range of this wave: 2^lsub cosets c = cstart | (gray(t) << lu) | u, t < 2^(lsub - lu), u < 2^lu with
 // lu = min(LU, lsub): binary inside a group of 2^lu cosets, whose bases are base(t) ^ ubase[u] (all gathered at once),
@@ -247,11 +258,13 @@ __global__ __launch_bounds__(kBThreads) void batch_step_kernel(const BInst* __re
     double nb = 0;
     constexpr int U = 1 << LU;
     const SR r(s);
-    Walk<LU> wk(it, s.cols, r.rhoc, s.nk, wave);
+    __shared__ uint64_t scols[40];
+    const uint64_t* cols = lds_cols(s.cols, s.nk, scols);
+    Walk<LU> wk(it, cols, r.rhoc, s.nk, wave);
     const uint64_t ng = wk.groups();
     const int ntq = r.ntq;
     for (uint64_t t = 0; t < ng; t++) {
-        if (t) wk.next(s.cols, r.rhoc, t);
+        if (t) wk.next(cols, r.rhoc, t);
         S v[U][NQ];
 #pragma unroll
         for (int u = 0; u < U; u++)                         // every load of the group first
@@ -304,11 +317,13 @@ __global__ __launch_bounds__(kBThreads) void batch_restore_kernel(const BInst* _
     if (!cp) return;
     constexpr int U = 1 << LU;
     const SR r(s);
-    Walk<LU> wk(it, s.cols, r.rhoc, s.nk, wave);
+    __shared__ uint64_t scols[40];
+    const uint64_t* cols = lds_cols(s.cols, s.nk, scols);
+    Walk<LU> wk(it, cols, r.rhoc, s.nk, wave);
     const uint64_t ng = wk.groups();
     const int ntq = r.ntq;
     for (uint64_t t = 0; t < ng; t++) {
-        if (t) wk.next(s.cols, r.rhoc, t);
+        if (t) wk.next(cols, r.rhoc, t);
         S v[U][NQ];
 #pragma unroll
         for (int u = 0; u < U; u++)
@@ -344,11 +359,13 @@ __global__ __launch_bounds__(kBThreads) void batch_back_kernel(const BInst* __re
     double2 g0 = make_double2(0, 0), g1 = g0;
     constexpr int U = 1 << LU;
     const SR r(s);
-    Walk<LU> wk(it, s.cols, r.rhoc, s.nk, wave);
+    __shared__ uint64_t scols[40];
+    const uint64_t* cols = lds_cols(s.cols, s.nk, scols);
+    Walk<LU> wk(it, cols, r.rhoc, s.nk, wave);
     const uint64_t ng = wk.groups();
     const int ntq = r.ntq;
     for (uint64_t t = 0; t < ng; t++) {
-        if (t) wk.next(s.cols, r.rhoc, t);
+        if (t) wk.next(cols, r.rhoc, t);
         S v[U][NQ], l0[U], l1[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {                       // every load of the group first

@@ -354,7 +354,7 @@ class BatchEngine:
 
     def score(self, bits, want_cond=True, want_amp=True):
         """bits: list (per plan) of int8 [lanes_i, ns_i].  Returns lists of log2p [lanes_i], cond [lanes_i, ns_i],
-        amp [lanes_i] complex."""
+        amp [lanes_i] complex (cond / amp entries are None when not requested)."""
         n = len(self.plans)
         bs = [np.ascontiguousarray(np.asarray(b, dtype=np.int8)) for b in bits]
         for b, p, ln in zip(bs, self.plans, self.lanes):
@@ -370,7 +370,8 @@ class BatchEngine:
         cdp = (DP * n)(*[_f8(x) for x in cd]) if want_cond else None
         amp = (DP * n)(*[_f8(x) for x in am]) if want_amp else None
         self._check(library().gs_batch_score(self._h, bp, lpp, cdp, amp))
-        return lp, cd, [a[:, 0] + 1j * a[:, 1] for a in am]
+        return (lp, cd if want_cond else [None] * n,
+                [a[:, 0] + 1j * a[:, 1] for a in am] if want_amp else [None] * n)
 
     def last_ms(self):
         """``(device ms of the last score, end of each step in ms after its start or -1)``."""
