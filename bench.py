@@ -62,6 +62,7 @@ def main():
     ap.add_argument("--c3-lanes", type=int, default=512)
     ap.add_argument("--c3-single", type=int, default=256, help="strings timed one per launch sequence")
     ap.add_argument("--c3-budget", type=float, default=120.0, help="GB of arena per batch in the mixed schedule")
+    ap.add_argument("--c3-mixed", type=int, default=1, help="run the 96-instance mixed schedule (0: skip)")
     ap.add_argument("--c5-samples", type=int, default=32)
     ap.add_argument("--c5-strings", type=int, default=256)
     ap.add_argument("--c5-c128", type=int, default=4, help="c5 samples/strings re-scored in a complex128 arena")
@@ -261,6 +262,8 @@ def workload_c3(gs, egs, args):
                       "speedup_vs_single": (N / tb) * t1,
                       "max_abs_dlog2p_vs_single": float(np.max(np.abs(lpb[:n1] - lp1)))}
     log(f"c3 batched: {N / tb:.0f} amplitudes/s")
+    if not args.c3_mixed:
+        return res
     # mixed instances
     plans, widths, bits = [], [], []
     for j, (n, band) in enumerate([(40, 16)] * 32 + [(56, 22)] * 32 + [(70, 26)] * 32):

@@ -39,6 +39,7 @@ constexpr int kLanes = 64;           // strings per wave (one lane group)
 constexpr int kWaves = 4;            // waves per block, each on its own part of the coset range
 constexpr int kRChunks = 64;         // chunks of the first reduction stage per lane group
 constexpr int kBThreads = kLanes * kWaves;
+constexpr int64_t kMinItems = 1024;  // work items (blocks) a step launch should have before tiles get coarser
 
 struct BStep {
     uint64_t cols[40];
@@ -178,6 +179,7 @@ struct Walk {
     }
     __device__ uint64_t groups() const { return on ? uint64_t(1) << (lsub - lu) : 0; }
     __device__ bool valid(int u) const { return u < (1 << lu); }
+    __device__ int own(int u) const { return u & ((1 << lu) - 1); }   // the coset whose addresses ubase[u] repeats
     __device__ void next(const uint64_t* __restrict__ cols, uint64_t rhoc, uint64_t t) {   // to group t >= 1
         const int j = lu + __builtin_ctzll(t);
         base ^= cols[j];
@@ -225,13 +227,14 @@ struct SR {
     }
 };
 
-// Routed signed slot sums A[tau][x] of one coset from its ntq slot values.
+// Routed signed slot sums A[tau][x] of one coset from its ntq slot values (ntq = NQ at compile time in the EX
+// variants, which have no branch on it).
 template <int NQ>
-__device__ __forceinline__ void route(const SR& s, const double2 (&v)[NQ], int pc, double2 (&A)[2][2]) {
-    A[0][0] = A[0][1] = A[1][0] = A[1][1] = make_double2(0, 0);
+__device__ __forceinline__ void route(const SR& s, const double2 (&v)[NQ], int pc, int ntq, double2 (&A)[2][2]) {
+    A[0][0] = A[0][1] = A[1][0] = A[1][1] = make_double2(-0.0, -0.0);   // x + (-0.0) = x: the first add folds away
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-        if (q >= s.ntq) break;
+        if (q >= ntq) break;
         const double2 w = v[q];
         const double2 ws = (pc ^ s.par[q]) ? cneg(w) : w;
         if (s.tau0[q]) A[1][0] = cadd(A[1][0], w); else A[0][0] = cadd(A[0][0], w);   // uniform branches
@@ -239,7 +242,7 @@ __device__ __forceinline__ void route(const SR& s, const double2 (&v)[NQ], int p
     }
 }
 
-template <class S, int NQ, int LU>
+template <class S, int NQ, int LU, bool EX>
 __global__ __launch_bounds__(kBThreads) void batch_step_kernel(const BInst* __restrict__ insts,
                                                                const BItem* __restrict__ items, BLaunch L) {
     const BItem it = items[blockIdx.x];
@@ -262,15 +265,19 @@ __global__ __launch_bounds__(kBThreads) void batch_step_kernel(const BInst* __re
     const uint64_t* cols = lds_cols(s.cols, s.nk, scols);
     Walk<LU> wk(it, cols, r.rhoc, s.nk, wave);
     const uint64_t ng = wk.groups();
-    const int ntq = r.ntq;
+    const int ntq = EX ? NQ : r.ntq;                      // EX: every instance has NQ slots at this step
     for (uint64_t t = 0; t < ng; t++) {
         if (t) wk.next(cols, r.rhoc, t);
         S v[U][NQ];
+        // Every load of the group first, and unconditionally: slots beyond this instance's ntq re-read slot 0, and the
+        // cosets u >= 2^lu of a short range re-read coset u mod 2^lu (ubase keeps only the low lu bits of u); both are
+        // discarded.  Loads (or their only uses) under branches leave a load possibly in flight at the loop latch,
+        // and the next group's load into the same registers then waits for all but the newest memory operation.
 #pragma unroll
-        for (int u = 0; u < U; u++)                         // every load of the group first
+        for (int u = 0; u < U; u++)
 #pragma unroll
             for (int q = 0; q < NQ; q++)
-                if (q < ntq && wk.valid(u)) v[u][q] = Sto<S>::ld(mem + (wk.base ^ wk.ubase[u] ^ r.off[q]) * B + b);
+                v[u][q] = Sto<S>::ld(mem + (wk.base ^ wk.ubase[u] ^ (q < ntq ? r.off[q] : 0)) * B + b);
         if (cp)
 #pragma unroll
             for (int u = 0; u < U; u++)
@@ -279,18 +286,18 @@ __global__ __launch_bounds__(kBThreads) void batch_step_kernel(const BInst* __re
                     if (q < ntq && wk.valid(u)) Sto<S>::st(cp + (wk.coset(t, u) * ntq + q) * B + b, v[u][q]);
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            if (!wk.valid(u)) continue;
             double2 vw[NQ];
 #pragma unroll
             for (int q = 0; q < NQ; q++) vw[q] = q < ntq ? Sto<S>::wide(v[u][q]) : make_double2(0, 0);
             double2 A[2][2];
-            route(r, vw, wk.pc ^ wk.upc[u], A);
+            route(r, vw, wk.pc ^ wk.upc[u], ntq, A);
             const uint64_t a0 = wk.base ^ wk.ubase[u];
             const double2 o0 = cfma(w0, A[0][0], cmul(w1, A[0][1]));
+            const double2 o1 = cfma(w0, A[1][0], cmul(w1, A[1][1]));
+            if (!wk.valid(u)) continue;                     // stores and norms of the lane's own cosets only
             Sto<S>::st(mem + a0 * B + b, Sto<S>::narrow(o0));
             nb += norm2(o0);
             if (r.m == 2) {
-                const double2 o1 = cfma(w0, A[1][0], cmul(w1, A[1][1]));
                 Sto<S>::st(mem + (a0 ^ r.g) * B + b, Sto<S>::narrow(o1));
                 nb += norm2(o1);
             }
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(kBThreads) void batch_step_kernel(const BInst* __re
 }
 
 // Put a compact copy of step k's input table back into the forward arena (restart of a checkpoint segment).
-template <class S, int NQ, int LU>
+template <class S, int NQ, int LU, bool EX>
 __global__ __launch_bounds__(kBThreads) void batch_restore_kernel(const BInst* __restrict__ insts,
                                                                   const BItem* __restrict__ items, BLaunch L) {
     const BItem it = items[blockIdx.x];
@@ -321,26 +328,28 @@ __global__ __launch_bounds__(kBThreads) void batch_restore_kernel(const BInst* _
     const uint64_t* cols = lds_cols(s.cols, s.nk, scols);
     Walk<LU> wk(it, cols, r.rhoc, s.nk, wave);
     const uint64_t ng = wk.groups();
-    const int ntq = r.ntq;
+    const int ntq = EX ? NQ : r.ntq;                      // EX: every instance has NQ slots at this step
     for (uint64_t t = 0; t < ng; t++) {
         if (t) wk.next(cols, r.rhoc, t);
         S v[U][NQ];
+        // No branches around the loads or their uses (see batch_step_kernel): a coset u >= 2^lu of a short range is
+        // coset u mod 2^lu, whose value it copies to the same address again.
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
             for (int q = 0; q < NQ; q++)
-                if (q < ntq && wk.valid(u)) v[u][q] = Sto<S>::ld(cp + (wk.coset(t, u) * ntq + q) * B + b);
+                if (q < ntq) v[u][q] = Sto<S>::ld(cp + (wk.coset(t, wk.own(u)) * ntq + q) * B + b);
 #pragma unroll
         for (int u = 0; u < U; u++)
 #pragma unroll
             for (int q = 0; q < NQ; q++)
-                if (q < ntq && wk.valid(u)) Sto<S>::st(mem + (wk.base ^ wk.ubase[u] ^ r.off[q]) * B + b, v[u][q]);
+                if (q < ntq) Sto<S>::st(mem + (wk.base ^ wk.ubase[u] ^ r.off[q]) * B + b, v[u][q]);
     }
 }
 
 // Backward step k: lambda_k (adjoint arena, at the coset's outputs) and the compact copy of P_{k-1} give
 // G_k(x) += sum_tau lambda[tau] A[tau][x]; lambda_{k-1} goes to the coset's input slots.
-template <class S, int NQ, int LU>
+template <class S, int NQ, int LU, bool EX>
 __global__ __launch_bounds__(kBThreads) void batch_back_kernel(const BInst* __restrict__ insts,
                                                                const BItem* __restrict__ items, BLaunch L) {
     const BItem it = items[blockIdx.x];
@@ -363,38 +372,43 @@ __global__ __launch_bounds__(kBThreads) void batch_back_kernel(const BInst* __re
     const uint64_t* cols = lds_cols(s.cols, s.nk, scols);
     Walk<LU> wk(it, cols, r.rhoc, s.nk, wave);
     const uint64_t ng = wk.groups();
-    const int ntq = r.ntq;
+    const int ntq = EX ? NQ : r.ntq;                      // EX: every instance has NQ slots at this step
     for (uint64_t t = 0; t < ng; t++) {
         if (t) wk.next(cols, r.rhoc, t);
         S v[U][NQ], l0[U], l1[U];
+        // Every load of the group first, with no branches around the loads or their uses (see batch_step_kernel): a
+        // coset u >= 2^lu of a short range reads coset u mod 2^lu and only its stores and gradient terms are dropped;
+        // a one-output step (g = 0) reads its output slot twice.
+        const uint64_t gq = r.m == 2 ? r.g : 0;
 #pragma unroll
-        for (int u = 0; u < U; u++) {                       // every load of the group first
-            if (!wk.valid(u)) continue;
+        for (int u = 0; u < U; u++) {
             const uint64_t a0 = wk.base ^ wk.ubase[u];
 #pragma unroll
             for (int q = 0; q < NQ; q++)
-                if (q < ntq) v[u][q] = Sto<S>::ld(cp + (wk.coset(t, u) * ntq + q) * B + b);
+                if (q < ntq) v[u][q] = Sto<S>::ld(cp + (wk.coset(t, wk.own(u)) * ntq + q) * B + b);
             l0[u] = Sto<S>::ld(lam + a0 * B + b);
-            if (r.m == 2) l1[u] = Sto<S>::ld(lam + (a0 ^ r.g) * B + b);
+            l1[u] = Sto<S>::ld(lam + (a0 ^ gq) * B + b);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            if (!wk.valid(u)) continue;
             const uint64_t a0 = wk.base ^ wk.ubase[u];
             const int pc = wk.pc ^ wk.upc[u];
             double2 vw[NQ];
 #pragma unroll
             for (int q = 0; q < NQ; q++) vw[q] = q < ntq ? Sto<S>::wide(v[u][q]) : make_double2(0, 0);
             double2 A[2][2];
-            route(r, vw, pc, A);
+            route(r, vw, pc, ntq, A);
             const double2 m0 = Sto<S>::wide(l0[u]);
             const double2 m1 = r.m == 2 ? Sto<S>::wide(l1[u]) : make_double2(0, 0);
-            g0 = cfma(m0, A[0][0], g0);
-            g1 = cfma(m0, A[0][1], g1);
+            double2 h0 = cfma(m0, A[0][0], g0);
+            double2 h1 = cfma(m0, A[0][1], g1);
             if (r.m == 2) {
-                g0 = cfma(m1, A[1][0], g0);
-                g1 = cfma(m1, A[1][1], g1);
+                h0 = cfma(m1, A[1][0], h0);
+                h1 = cfma(m1, A[1][1], h1);
             }
+            if (!wk.valid(u)) continue;                     // gradient terms and stores of the lane's own cosets only
+            g0 = h0;
+            g1 = h1;
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
                 if (q >= ntq) break;
@@ -540,6 +554,7 @@ struct gs_batch {
     std::vector<std::vector<int>> lg;     // [inst][step]
     int lg_default = 8;                   // smallest tile (sizes the partial buffers); steps start at 2^9
     std::vector<int> nq;                  // per step: most slots per coset among the instances (kernel variant)
+    std::vector<int> nq_exact;            // per step: 1 if every instance at the step has nq slots per coset
     int* dcnt = nullptr;                  // device [maxns][ninst]: items per lane group of instance i at step k
     int maxgroups = 1;
     int lu = 2;                           // log2 cosets a lane gathers at once
@@ -582,13 +597,31 @@ int build_items(gs_batch* bt) {
     bt->ditems.assign(size_t(bt->maxns), nullptr);
     bt->nitems.assign(size_t(bt->maxns), 0);
     bt->nq.assign(size_t(bt->maxns), 1);
+    bt->nq_exact.assign(size_t(bt->maxns), 0);
     for (int k = 0; k < bt->maxns; k++) {
+        // Tiles set per step index (by hand or by the autotuner, often on a larger batch) are made finer for this
+        // step, down to 2^lg_default cosets, while the step has fewer than kMinItems items: one item per block, and
+        // a launch of a few hundred blocks leaves most of the chip idle while each wave walks a long coset range.
+        int shrink = 0;
+        for (;; shrink++) {
+            int64_t tot = 0;
+            bool finer = false;
+            for (size_t i = 0; i < bt->plans.size(); i++) {
+                const gs::Plan& P = bt->plans[i];
+                if (k >= P.ns) continue;
+                const int nk = P.steps[size_t(k)].nk;
+                const int lg = std::min(nk, std::max(bt->lg[i][size_t(k)] - shrink, std::min(nk, bt->lg_default)));
+                tot += int64_t(bt->B[i] / kLanes) << (nk - lg);
+                finer = finer || lg > std::min(nk, bt->lg_default);
+            }
+            if (tot >= kMinItems || !finer) break;
+        }
         std::vector<BItem> items;
         for (size_t i = 0; i < bt->plans.size(); i++) {
             const gs::Plan& P = bt->plans[i];
             if (k >= P.ns) continue;
             const int nk = P.steps[size_t(k)].nk;
-            const int lg = std::min(nk, bt->lg[i][size_t(k)]);
+            const int lg = std::min(nk, std::max(bt->lg[i][size_t(k)] - shrink, std::min(nk, bt->lg_default)));
             const int count = 1 << (nk - lg);
             cnt[size_t(k) * ni + i] = count;
             for (int g = 0; g < bt->B[i] / kLanes; g++)
@@ -607,6 +640,10 @@ int build_items(gs_batch* bt) {
         for (size_t i = 0; i < bt->plans.size(); i++)
             if (k < bt->plans[i].ns) nqk = std::max(nqk, bt->plans[i].steps[size_t(k)].ntq);
         bt->nq[size_t(k)] = nqk;
+        int exact = 1;
+        for (size_t i = 0; i < bt->plans.size(); i++)
+            if (k < bt->plans[i].ns && bt->plans[i].steps[size_t(k)].ntq != nqk) exact = 0;
+        bt->nq_exact[size_t(k)] = exact;
         bt->nitems[size_t(k)] = int(items.size());
         void* d = nullptr;
         if (hipMalloc(&d, std::max<size_t>(items.size() * sizeof(BItem), 16)) != hipSuccess)
@@ -635,22 +672,17 @@ int max_items(const gs_batch* bt, size_t i) {
     return m;
 }
 
-// Instantiation of kernel K<S, NQ, LU> for NQ in {1, 2, 4}, LU in 0..3.
-#define GS_BATCH_VARIANT(K, S, nq, lu)                                                                            \
+// Instantiation of kernel K<S, NQ, LU, EX> for NQ in {1, 2, 4}, LU in 0..3, EX (every instance has NQ slots at the
+// step) false or true.
+#define GS_BV(K, S, nq, lu)                                                                                        \
+    case (nq) * 8 + (lu) * 2: return reinterpret_cast<const void*>(&K<S, nq, lu, false>);                          \
+    case (nq) * 8 + (lu) * 2 + 1: return reinterpret_cast<const void*>(&K<S, nq, lu, true>);
+#define GS_BATCH_VARIANT(K, S, nq, lu, ex)                                                                        \
     [&]() -> const void* {                                                                                         \
-        switch ((nq) * 4 + (lu)) {                                                                                 \
-            case 4: return reinterpret_cast<const void*>(&K<S, 1, 0>);                                             \
-            case 5: return reinterpret_cast<const void*>(&K<S, 1, 1>);                                             \
-            case 6: return reinterpret_cast<const void*>(&K<S, 1, 2>);                                             \
-            case 7: return reinterpret_cast<const void*>(&K<S, 1, 3>);                                             \
-            case 8: return reinterpret_cast<const void*>(&K<S, 2, 0>);                                             \
-            case 9: return reinterpret_cast<const void*>(&K<S, 2, 1>);                                             \
-            case 10: return reinterpret_cast<const void*>(&K<S, 2, 2>);                                            \
-            case 11: return reinterpret_cast<const void*>(&K<S, 2, 3>);                                            \
-            case 16: return reinterpret_cast<const void*>(&K<S, 4, 0>);                                            \
-            case 17: return reinterpret_cast<const void*>(&K<S, 4, 1>);                                            \
-            case 18: return reinterpret_cast<const void*>(&K<S, 4, 2>);                                            \
-            case 19: return reinterpret_cast<const void*>(&K<S, 4, 3>);                                            \
+        switch ((nq) * 8 + (lu) * 2 + ((ex) ? 1 : 0)) {                                                           \
+            GS_BV(K, S, 1, 0) GS_BV(K, S, 1, 1) GS_BV(K, S, 1, 2) GS_BV(K, S, 1, 3)                                 \
+            GS_BV(K, S, 2, 0) GS_BV(K, S, 2, 1) GS_BV(K, S, 2, 2) GS_BV(K, S, 2, 3)                                 \
+            GS_BV(K, S, 4, 0) GS_BV(K, S, 4, 1) GS_BV(K, S, 4, 2) GS_BV(K, S, 4, 3)                                 \
         }                                                                                                          \
         return nullptr;                                                                                            \
     }()
@@ -662,9 +694,10 @@ int launch_batch(gs_batch* bt, BKernel kind, int k, BLaunch L) {
     const int n = bt->nitems[size_t(k)];
     if (n == 0) return 0;
     const int nq = bt->nq[size_t(k)], lu = bt->lu;
-    const void* fn = kind == kBStep ? GS_BATCH_VARIANT(batch_step_kernel, S, nq, lu)
-                   : kind == kBRestore ? GS_BATCH_VARIANT(batch_restore_kernel, S, nq, lu)
-                                       : GS_BATCH_VARIANT(batch_back_kernel, S, nq, lu);
+    const bool ex = bt->nq_exact[size_t(k)] != 0;
+    const void* fn = kind == kBStep ? GS_BATCH_VARIANT(batch_step_kernel, S, nq, lu, ex)
+                   : kind == kBRestore ? GS_BATCH_VARIANT(batch_restore_kernel, S, nq, lu, ex)
+                                       : GS_BATCH_VARIANT(batch_back_kernel, S, nq, lu, ex);
     if (!fn) return bfail(16, "no kernel variant");
     const BInst* insts = bt->dinst;
     const BItem* items = bt->ditems[size_t(k)];
