@@ -10,6 +10,8 @@ namespace gs {
 constexpr int kThreads = 256;       // threads per sweep block (4 waves of 64)
 constexpr int kMaxGridLog2 = 14;    // at most 16384 blocks per sweep; beyond that each block loops over tiles
 constexpr int kDefaultGridLog2 = 9;     // measured best for the width-30 sweeps (a few waves per SIMD suffice)
+constexpr int kDefaultGridLog2One = 12; // plain sweeps of one-slot steps: one load per thread and tile, so more waves
+                                        // (width 34, 1-slot norm passes: 691 -> 343 ms per sample at 2^12 blocks)
 constexpr int kFenceBlocks = 64;    // sweeps of at most this many blocks reduce in their last block
 constexpr int kPartStride = 8;      // doubles per block partial (write norms + fused next-step norms)
 constexpr size_t kPartDoubles = size_t(kPartStride) << kMaxGridLog2;
@@ -40,7 +42,8 @@ struct KStep {
     uint32_t parmask;      // bit j: parity(qv_j & rho) of offset j
     int nk;                // number of coset coordinates
     int lr;                // log2 run length (cosets per thread along unit bits)
-    int vg;                // 1, or 2 when coordinate lr is the pairing coordinate of the next step
+    int vg;                // 1, or 2 when coordinate lr is a per-thread coordinate too: the pairing coordinate of the
+                           // next step (fused sweeps), or a second run per thread (plain sweeps of 1- and 2-slot steps)
     int lg_grid;           // log2 blocks
     int lg_iter;           // log2 tiles per block
     int ntq, m, route;

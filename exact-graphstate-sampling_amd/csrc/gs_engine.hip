@@ -247,9 +247,12 @@ struct gs_engine {
     double last_ms = 0;
     bool allow_fusion = true;
     int grid_cap = kDefaultGridLog2;  // log2 of the most blocks a sweep launches
+    int grid_cap1 = kDefaultGridLog2One;  // ... a plain sweep of a one-slot step
     int small_nk = 10;              // runs of steps with nk <= small_nk go to one single-block launch (-1: never)
     int walk = 0;                   // tile order: 0 forward, 1 reverse on odd steps, 2 reverse on every step
     int fused_run = -1;             // log2 run of the fused layout (-1: the storage's default)
+    int pair_ntq = 0;               // plain sweeps of steps with at most this many slots per coset take two runs per thread
+    int pair_far = 0;               // ... the second one at the top coordinate (1) or the next unit bit (0)
     // optional per-sweep timing (bench / profiling)
     bool profile = false;
     std::vector<hipEvent_t> pev;
@@ -630,12 +633,25 @@ int build_steps(gs_engine* e) {
         coord_basis(si.cols, {}, 0, basis);
         apply_basis(si.cols, si.rhoc, basis, ks);
         ks.lr = unit_lead(ks) >= lrmax ? lrmax : 0;
-        set_geometry(ks, ks.lr, e->grid_cap);
+        // Optionally (off by default) steps with few slots per coset give each thread a second run (coordinate lr
+        // becomes a thread coordinate), the top coordinate (pair_far: a distant region of the table) or the next unit
+        // bit.  Measured on one MI355X: adjacent pairs make the width-30 samples slower (4.47 -> 4.57 ms) and the
+        // width-34 ones faster at 2^9 blocks (2.08 -> 1.95 s) but slower at 2^12 (1.66 -> 1.90 s), distant pairs
+        // slower everywhere (width 34: 3.6 s); a one-slot step gets its loads in flight from more blocks instead.
+        if (ks.lr == lrmax && ks.lr > 0 && ks.ntq <= e->pair_ntq && ks.nk >= ks.lr + 13) {
+            ks.vg = 2;
+            if (e->pair_far) {
+                const int a = ks.lr, b = ks.nk - 1;
+                std::swap(ks.cols[a], ks.cols[b]);
+                if (((ks.pmask[0] >> a) ^ (ks.pmask[0] >> b)) & 1) ks.pmask[0] ^= (uint64_t(1) << a) | (uint64_t(1) << b);
+            }
+        }
+        set_geometry(ks, ks.lr + (ks.vg == 2 ? 1 : 0), ks.ntq == 1 ? e->grid_cap1 : e->grid_cap);
         auto& L = e->launch[size_t(k)];
         L.grid = 1 << ks.lg_grid;
-        L.gram = sweep_kernel_for(e->storage, ks.lr, 1, ks.ntq, ks.route, kGram, 0);
-        L.direct = sweep_kernel_for(e->storage, ks.lr, 1, ks.ntq, ks.route, kDirect, 0);
-        L.write = sweep_kernel_for(e->storage, ks.lr, 1, ks.ntq, ks.route, kWrite, 0);
+        L.gram = sweep_kernel_for(e->storage, ks.lr, ks.vg, ks.ntq, ks.route, kGram, 0);
+        L.direct = sweep_kernel_for(e->storage, ks.lr, ks.vg, ks.ntq, ks.route, kDirect, 0);
+        L.write = sweep_kernel_for(e->storage, ks.lr, ks.vg, ks.ntq, ks.route, kWrite, 0);
         if (!L.gram || !L.direct || !L.write) return fail(15, "no kernel for step " + std::to_string(k));
     }
     for (int k = 0; k < P.ns; k++) e->hplain[size_t(k)].rev = e->walk == 2 || (e->walk == 1 && (k & 1));
@@ -1184,17 +1200,20 @@ int gs_step_geometry(const gs_engine* eng, int k, int* out) {
     out[7] = s.lg_iter;
     out[8] = eng->fusable[size_t(k)] ? f.lr : -1;
     out[9] = eng->fusable[size_t(k)] ? f.lg_iter : -1;
+    out[10] = s.vg;
     return 0;
 }
 
-// Cap the blocks of one sweep at 2^lg (1..14, default 9); tiles beyond the cap loop inside each block.  Rebuilds
-// and uploads the step descriptors (synchronous).
-int gs_set_grid_log2(gs_engine* eng, int lg) {
+// Cap the blocks of one sweep at 2^lg (1..14, default 9, and 12 for the plain sweeps of one-slot steps); tiles
+// beyond the cap loop inside each block.  Sets both caps to lg (one_slot_only: the one-slot cap alone).  Rebuilds and
+// uploads the step descriptors (synchronous).
+static int set_grid_caps(gs_engine* eng, int lg, bool one_slot_only) {
     if (!eng) return fail(16, "null engine");
     if (lg < 1 || lg > kMaxGridLog2) return fail(16, "grid log2 must be in 1.." + std::to_string(kMaxGridLog2));
     HIP_TRY(hipSetDevice(eng->device));
     HIP_TRY(hipStreamSynchronize(eng->stream));
-    eng->grid_cap = lg;
+    if (!one_slot_only) eng->grid_cap = lg;
+    eng->grid_cap1 = lg;
     if (int rc = build_steps(eng)) return rc;
     const size_t ns = size_t(eng->plan.ns);
     if (ns) {
@@ -1204,6 +1223,9 @@ int gs_set_grid_log2(gs_engine* eng, int lg) {
     }
     return 0;
 }
+
+int gs_set_grid_log2(gs_engine* eng, int lg) { return set_grid_caps(eng, lg, false); }
+int gs_set_grid_log2_one_slot(gs_engine* eng, int lg) { return set_grid_caps(eng, lg, true); }
 
 // Coset-coordinate columns of step k as the sweeps use them (fused: the fused layout), into cols[0 .. cap-1];
 // returns nk, or -1 (no such step / layout).
@@ -1223,6 +1245,25 @@ int gs_set_fused_run(gs_engine* eng, int lr) {
     HIP_TRY(hipSetDevice(eng->device));
     HIP_TRY(hipStreamSynchronize(eng->stream));
     eng->fused_run = lr;
+    if (int rc = build_steps(eng)) return rc;
+    const size_t ns = size_t(eng->plan.ns);
+    if (ns) {
+        HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+// Plain sweeps of steps with at most ntq_max slots per coset (0: none, at most 2) take two runs per thread, the second
+// at the top coordinate (far = 1) or at the next unit bit (far = 0); results do not depend on it.  Rebuilds the step
+// descriptors.
+int gs_set_plain_pairs(gs_engine* eng, int ntq_max, int far) {
+    if (!eng) return fail(16, "null engine");
+    if (ntq_max < 0 || ntq_max > 2 || far < 0 || far > 1) return fail(16, "ntq_max must be 0..2 and far 0 or 1");
+    HIP_TRY(hipSetDevice(eng->device));
+    HIP_TRY(hipStreamSynchronize(eng->stream));
+    eng->pair_ntq = ntq_max;
+    eng->pair_far = far;
     if (int rc = build_steps(eng)) return rc;
     const size_t ns = size_t(eng->plan.ns);
     if (ns) {

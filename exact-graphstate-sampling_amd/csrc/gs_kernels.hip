@@ -991,6 +991,12 @@ void* by_layout(int lr, int vg, int ntq, int route, int route2) {
         }
         else return lr == 0 ? by_ntq<S, 1, 2, SWEEP>(ntq, route, route2) : nullptr;
     } else {
+        if (vg == 2) {                           // two runs per thread: plain sweeps of one- and two-slot steps
+            if (lr != 1) return nullptr;
+            if (ntq == 1) return by_route<S, 2, 2, 1, SWEEP>(route, route2);
+            if (ntq == 2) return by_route<S, 2, 2, 2, SWEEP>(route, route2);
+            return nullptr;
+        }
         if (vg != 1) return nullptr;
         if (lr == 0) return by_ntq<S, 1, 1, SWEEP>(ntq, route, route2);
         return lr == 1 ? by_ntq<S, 2, 1, SWEEP>(ntq, route, route2) : nullptr;

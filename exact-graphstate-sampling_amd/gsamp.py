@@ -21,7 +21,7 @@ SEED_PLAIN, SEED_SEQ = 0, 1
 ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy", "gs_engine_create",
                "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
                "gs_last_device_ms", "gs_last_error", "gs_profile_enable", "gs_profile_read",
-               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_small_steps", "gs_set_walk", "gs_set_fused_run", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
+               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_grid_log2_one_slot", "gs_set_small_steps", "gs_set_walk", "gs_set_plain_pairs", "gs_set_fused_run", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
                "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms", "gs_tt_plan_load", "gs_tt_plan_info",
                "gs_tt_plan_destroy", "gs_tt_engine_create", "gs_tt_engine_destroy", "gs_tt_prob", "gs_tt_chain",
@@ -74,7 +74,9 @@ def library():
         "gs_step_columns": (I, [P, I, I, ctypes.POINTER(ctypes.c_uint64), I]),
         "gs_set_grid_log2": (I, [P, I]),
         "gs_set_small_steps": (I, [P, I]),
+        "gs_set_grid_log2_one_slot": (I, [P, I]),
         "gs_set_walk": (I, [P, I]),
+        "gs_set_plain_pairs": (I, [P, I, I]),
         "gs_set_fused_run": (I, [P, I]),
         "gs_copy_probe": (I, [I, I64, dp]),
         "gs_batch_create": (I, [ctypes.POINTER(P), ip, I, I, I, pp]),
@@ -273,10 +275,10 @@ class Engine:
         return ms.reshape(self.ns, 5), n.reshape(self.ns, 5)
 
     def geometry(self, k):
-        """Launch geometry of step k: dict(lr, grid, ntq, route, fusable, fgrid, route2, lg_iter)."""
-        out = np.zeros(10, np.int32)
+        """Launch geometry of step k: dict(lr, grid, ntq, route, fusable, fgrid, route2, lg_iter, flr, flg_iter, vg)."""
+        out = np.zeros(11, np.int32)
         _check(library().gs_step_geometry(self._h, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
-        return dict(zip(("lr", "grid", "ntq", "route", "fusable", "fgrid", "route2", "lg_iter", "flr", "flg_iter"),
+        return dict(zip(("lr", "grid", "ntq", "route", "fusable", "fgrid", "route2", "lg_iter", "flr", "flg_iter", "vg"),
                         (int(x) for x in out)))
 
     def columns(self, k, fused=False):
@@ -297,12 +299,22 @@ class Engine:
         """Tile order of the sweeps: 0 forward, 1 reverse on odd steps, 2 reverse on every step."""
         _check(library().gs_set_walk(self._h, int(mode)))
 
+    def set_plain_pairs(self, ntq_max, far=0):
+        """Plain sweeps of steps with at most ntq_max slots per coset (0..2; default 0, off) take two runs per thread,
+        the second at the top coordinate (far=1) or at the next unit bit (far=0)."""
+        _check(library().gs_set_plain_pairs(self._h, int(ntq_max), int(far)))
+
     def set_small_steps(self, nk):
         """Run consecutive steps of at most 2^nk cosets in one single-block launch (-1: never)."""
         _check(library().gs_set_small_steps(self._h, int(nk)))
 
+    def set_grid_log2_one_slot(self, lg):
+        """Cap the plain sweeps of one-slot steps alone at 2^lg blocks (1..14, default 12)."""
+        _check(library().gs_set_grid_log2_one_slot(self._h, int(lg)))
+
     def set_grid_log2(self, lg):
-        """Cap every sweep at 2^lg blocks (1..14, default 9); tiles beyond the cap loop inside each block."""
+        """Cap every sweep at 2^lg blocks (1..14, default 9; one-slot plain sweeps 12); tiles beyond the cap loop
+        inside each block.  Sets the one-slot cap to lg as well."""
         _check(library().gs_set_grid_log2(self._h, int(lg)))
 
 

@@ -15,8 +15,10 @@ import sys
 
 
 def _short(name):
-    return name.split("(gs::KStep")[0].replace("HIP_vector_type<float, 2u>", "float2") \
+    """Kernel name without its parameter list, with the vector types spelled float2 / double2."""
+    n = name.replace("(anonymous namespace)::", "").replace("HIP_vector_type<float, 2u>", "float2") \
         .replace("HIP_vector_type<double, 2u>", "double2").replace("void ", "")
+    return n.split(">(")[0] + ">" if ">(" in n else n.split("(")[0]
 
 
 def stats(trace_dir, top=14):

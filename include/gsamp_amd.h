@@ -151,21 +151,27 @@ double gs_last_device_ms(const gs_engine* eng);
  * (sweeps: 0 Gram norms, 1 direct norms, 2 write, 3 write + next Gram, 4 write + next direct). */
 int gs_profile_enable(gs_engine* eng, int on);
 int gs_profile_read(const gs_engine* eng, double* ms, int64_t* launches);
-/* Launch geometry of step k, out[10]: log2 cosets per run, blocks, slots per coset, route, fusable, blocks of the
+/* Launch geometry of step k, out[11]: log2 cosets per run, blocks, slots per coset, route, fusable, blocks of the
  * fused sweep, next step's route, log2 tiles per block, log2 run of the fused layout, its log2 tiles per block
- * (the last two -1 when the step does not fuse). */
+ * (those two -1 when the step does not fuse), runs per thread of the plain sweeps (1 or 2). */
 int gs_step_geometry(const gs_engine* eng, int k, int* out);
 /* Coset-coordinate columns (physical address vectors) of step k as the plain (fused = 0) or fused sweep uses them,
  * into cols[0 .. cap-1]; returns nk, or -1 when the step or layout does not exist. */
 int gs_step_columns(const gs_engine* eng, int k, int fused, uint64_t* cols, int cap);
 /* Fold the next step's norm pass into the previous write (on, default) or run it as its own sweep (off). */
 int gs_set_fusion(gs_engine* eng, int on);
-/* Cap one sweep at 2^lg blocks (1..14, default 9); tiles beyond the cap loop inside each block. */
+/* Cap one sweep at 2^lg blocks (1..14, default 9; plain sweeps of one-slot steps 12); tiles beyond the cap loop inside
+   each block.  Sets the one-slot cap to lg as well. */
 int gs_set_grid_log2(gs_engine* eng, int lg);
+/* Cap the plain sweeps of one-slot steps alone at 2^lg blocks (1..14, default 12). */
+int gs_set_grid_log2_one_slot(gs_engine* eng, int lg);
 /* log2 run length of the fused layout (complex64: 1 default or 2; complex128: 0; -1 restores the default). */
 int gs_set_fused_run(gs_engine* eng, int lr);
 /* Tile order of the sweeps: 0 forward (default), 1 reverse on odd steps, 2 reverse on every step. */
 int gs_set_walk(gs_engine* eng, int mode);
+/* Plain sweeps of steps with at most ntq_max slots per coset (0..2; default 0, off) take two runs per thread, the
+   second at the top coordinate (far = 1) or at the next unit bit (far = 0). */
+int gs_set_plain_pairs(gs_engine* eng, int ntq_max, int far);
 /* Runs of two or more steps with at most 2^nk cosets go to one single-block launch (nk -1: never; default 10). */
 int gs_set_small_steps(gs_engine* eng, int nk);
 /* One plain device copy of `bytes` (read + write), *ms its duration: calibrates hardware byte counters. */

@@ -268,6 +268,43 @@ def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2, walk, run):
     assert close(lpp, lp, 0) < tol
 
 
+@pytest.mark.parametrize("pair_ntq,far,fusion", [(0, 0, True), (1, 1, True), (1, 0, False), (2, 1, False),
+                                                 (2, 0, True)])
+def test_c4_plain_pairs_vs_oracle(c4_plan, tmp_path, pair_ntq, far, fusion):
+    """n=60 band-24 instance in the FP64 arena: samples, prob and chain equal the oracle (bits identical, 1e-12)
+    whatever the plain sweeps' layout -- one run per thread, or a second run at the next unit bit or at the top
+    coordinate for one-slot (and two-slot) steps -- with the norm passes folded into the writes or all plain."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    import subprocess
+    ora = os.path.join(root, "oracle", "gs_oracle")
+    out = subprocess.run([ora, "--c128", c4_plan, "sample", "3", "778"], capture_output=True, text=True,
+                         check=True).stdout.split("\n")
+    ref = [l.split() for l in out if l.strip()]
+    print(f"c4 sample 3 778 plain pairs ntq<={pair_ntq} far={far} fusion={fusion}", flush=True)
+    with gs.Engine(c4_plan, gs.C128) as e:
+        e.set_plain_pairs(pair_ntq, far)
+        e.set_fusion(fusion)
+        vgs = [e.geometry(k)["vg"] for k in range(e.ns)]
+        ntqs = [e.geometry(k)["ntq"] for k in range(e.ns)]
+        if pair_ntq:
+            assert any(v == 2 for v in vgs)
+        assert all(v == 1 or n <= pair_ntq for v, n in zip(vgs, ntqs))
+        bits, lp = e.sample(3, 778)
+        lpp = e.prob(bits)
+        lpc, cond, _ = e.chain(bits)
+    assert [egs.planio.format_bits(b) for b in bits] == [r[0] for r in ref]
+    assert close(lp, [float(r[1]) for r in ref], 0) < TOL[gs.C128]
+    assert close(lpp, lp, 0) < TOL[gs.C128]
+    assert close(lpc, lp, 0) < TOL[gs.C128]
+    strings = tmp_path / "strings.txt"
+    strings.write_text("\n".join(r[0] for r in ref) + "\n")
+    chain_ref = subprocess.run([ora, "--c128", c4_plan, "chain", str(strings)], capture_output=True, text=True,
+                               check=True).stdout.split("\n")
+    cref = [l.split() for l in chain_ref if l.strip()]
+    assert close(lpc, [float(r[0]) for r in cref], 0) < TOL[gs.C128]
+    assert close(cond, [[float(x) for x in r[1:]] for r in cref], 0) < 2e-9      # printed with %.9f
+
+
 @pytest.mark.parametrize("storage", [gs.C64, gs.C128])
 def test_sliced_prefix_resume_with_twist(tmp_path, storage):
     """Two slice vertices with suffix neighbours on the n=24 band-10 instance: the four slice tables of step K=18
