@@ -26,7 +26,9 @@ enum Route { kRoute1 = 0, kRouteSplit = 1, kRouteMixA = 2, kRouteMixB = 3 };
 
 // Sweep kinds.  kWriteGram / kWriteDirect store step k and, from the values just stored, reduce the norms of step
 // k+1 (whose two slots per coset are both produced by one thread: the thread's cosets span the pairing vector).
-enum Sweep { kGram = 0, kDirect = 1, kWrite = 2, kWriteGram = 3, kWriteDirect = 4 };
+enum Sweep { kGram = 0, kDirect = 1, kWrite = 2, kWriteGram = 3, kWriteDirect = 4,
+             kWriteGram1 = 5 };   // kWriteGram1: a plain write folding in the Gram of a one-slot next step;
+                                  // the engine launches it as its kWriteGram (same partials and chain logic)
 
 // One step as the sweep kernels see it (device memory, read with uniform loads).  Coset coordinates are in the
 // kernel's order: bits 0..lr-1 are unit physical bits (a run of consecutive slots), bit lr is the pairing coordinate
@@ -53,7 +55,7 @@ struct KStep {
     int eps;               // fused: the partner of output tau is output tau ^ eps of the paired coset
     int phig, nug;         // fused: slot-index and sign parity of g
     int rev;               // walk the tiles of each block in reverse Gray order
-    int pad;
+    int sgg, sgk;          // kWriteGram1: next step's sign parity of g, and its constant (pmask[2]: per coordinate)
 };
 
 // One step of the two-table (gather-form) engine as its kernels see it: output cp of the new table gathers the inputs

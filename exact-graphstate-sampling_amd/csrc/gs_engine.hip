@@ -599,6 +599,42 @@ bool build_fused(const gs_engine* e, int k, KStep& f) {
     return true;
 }
 
+// Fused descriptor of step k when step k+1 is a one-slot, one-output step (kWriteGram1): every value step k's plain
+// write stores is then the one slot of one coset of step k+1, whose x = 1 sign (-1)^(pc ^ par) is an affine function
+// of the stored address, so the write also sums step k+1's Gram matrix (G00 = G11 = sum |v|^2, Re G01 = signed sum).
+// Requires the live table after step k to be exactly step k+1's slot space.
+bool build_fused1(const gs_engine* e, int k, KStep& f) {
+    const Plan& P = e->plan;
+    if (k + 1 >= P.ns) return false;
+    const StepInfo& s0 = P.steps[size_t(k)];
+    const StepInfo& s1 = P.steps[size_t(k + 1)];
+    const KStep& p0 = e->hplain[size_t(k)];
+    if (s1.ntq != 1 || s1.m != 1 || s1.growth || s1.tau0[0] != 0 || s1.tau1[0] != 0) return false;
+    if (p0.vg != 1 || p0.lr != 1) return false;
+    if (s0.nk + (s0.m == 2 ? 1 : 0) != s1.nk) return false;
+    // parity(c1 & rhoc1) of the step-(k+1) coordinates c1 of an address in its slot space (linear in the address)
+    auto sign_of = [&](uint64_t x, int& out) {
+        uint64_t c = 0;
+        if (!gf2_solve(s1.cols, x, c)) return false;
+        out = parity64(c & s1.rhoc);
+        return true;
+    };
+    int kap = 0;
+    if (!sign_of(s1.off[0], kap)) return false;
+    f = p0;
+    f.pmask[1] = f.pmask[2] = 0;
+    for (int j = 0; j < p0.nk; j++) {
+        int b = 0;
+        if (!sign_of(p0.cols[j], b)) return false;            // a live column outside the next step's slot space
+        f.pmask[2] |= uint64_t(b) << j;
+    }
+    int sg = 0;
+    if (s0.m == 2 && !sign_of(s0.g, sg)) return false;
+    f.sgg = sg;
+    f.sgk = kap ^ s1.par[0];                                  // slot a = base1(c1) ^ off: sign of c1 = sign(a) ^ sign(off)
+    return true;
+}
+
 int build_steps(gs_engine* e) {
     const Plan& P = e->plan;
     e->hplain.assign(size_t(P.ns), KStep{});
@@ -657,11 +693,21 @@ int build_steps(gs_engine* e) {
     for (int k = 0; k < P.ns; k++) e->hplain[size_t(k)].rev = e->walk == 2 || (e->walk == 1 && (k & 1));
     for (int k = 0; k + 1 < P.ns; k++) {
         KStep f;
-        if (!build_fused(e, k, f)) continue;
         auto& L = e->launch[size_t(k)];
-        L.wgram = sweep_kernel_for(e->storage, f.lr, 2, f.ntq, f.route, kWriteGram, f.route2);
-        L.wdirect = sweep_kernel_for(e->storage, f.lr, 2, f.ntq, f.route, kWriteDirect, f.route2);
-        if (!L.wgram || !L.wdirect) continue;
+        if (build_fused(e, k, f)) {                       // next step has two slots: pairing layout
+            L.wgram = sweep_kernel_for(e->storage, f.lr, 2, f.ntq, f.route, kWriteGram, f.route2);
+            L.wdirect = sweep_kernel_for(e->storage, f.lr, 2, f.ntq, f.route, kWriteDirect, f.route2);
+            if (!L.wgram || !L.wdirect) {
+                L.wgram = L.wdirect = nullptr;
+                continue;
+            }
+        } else if (build_fused1(e, k, f)) {               // next step has one slot: the plain layout (Gram form only)
+            L.wgram = sweep_kernel_for(e->storage, f.lr, 1, f.ntq, f.route, kWriteGram1, 0);
+            L.wdirect = nullptr;
+            if (!L.wgram) continue;
+        } else {
+            continue;
+        }
         L.fgrid = 1 << f.lg_grid;
         e->hfused[size_t(k)] = f;
         e->fusable[size_t(k)] = 1;

@@ -549,7 +549,9 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     constexpr int V = VR * VG;
     constexpr bool WRITES = SWEEP >= kWrite;
     constexpr bool FUSED = SWEEP == kWriteGram || SWEEP == kWriteDirect;
-    constexpr int NA = SWEEP == kGram ? 4 : SWEEP == kDirect ? 2 : SWEEP == kWrite ? 2 : SWEEP == kWriteGram ? 6 : 4;
+    constexpr bool GRAM1 = SWEEP == kWriteGram1;
+    constexpr int NA = SWEEP == kGram ? 4 : SWEEP == kDirect ? 2 : SWEEP == kWrite ? 2
+                     : SWEEP == kWriteGram || SWEEP == kWriteGram1 ? 6 : 4;
     if (ctl->dead) return;
     const KStep& s = *sp;
     const int nk = s.nk;
@@ -613,6 +615,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     const uint32_t parmask = s.parmask;
     const uint64_t g = s.g;
     const int eps = s.eps, phig = s.phig, nug = s.nug;
+    const int sgg = s.sgg, sgk = s.sgk;
     const uint32_t par2mask = uint32_t(s.par2) << 1;
 
     double acc[NA];
@@ -685,7 +688,16 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
                         // table's norm is read only by a growth step's analytic norms, and a fused write is never
                         // followed by a growth step
                         add_norm(acc[0], v);
-                        if constexpr (!FUSED) add_norm(acc[1], Cx<S>::widen(sv));
+                        if constexpr (GRAM1) {
+                            // the next step reads this value as its coset's one slot: |v|^2 into N = G00 = G11 and,
+                            // signed by that coset's (-1)^(pc ^ par), into Re G01 (Im G01 = 0)
+                            const double nv = norm2(Cx<S>::widen(sv));
+                            const uint32_t sg = uint32_t(((pbase ^ plow[i]) >> 2) ^ (t ? sgg : 0) ^ sgk) & 1;
+                            acc[2] += nv;
+                            acc[4] += flip(nv, sg);
+                        } else if constexpr (!FUSED) {
+                            add_norm(acc[1], Cx<S>::widen(sv));
+                        }
                     }
                 }
             }
@@ -731,7 +743,8 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
         }
     }
     if constexpr (SWEEP == kWriteGram) gram_assemble<ROUTE2>(gi, s.par2, acc + 2);
-    sweep_epilogue<NA>(acc, part, ctl, epi, SWEEP);
+    if constexpr (GRAM1) acc[3] = acc[2];
+    sweep_epilogue<NA>(acc, part, ctl, epi, GRAM1 ? int(kWriteGram) : SWEEP);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -982,6 +995,9 @@ void* by_ntq(int ntq, int route, int route2) {
 template <class S, int SWEEP>
 void* by_layout(int lr, int vg, int ntq, int route, int route2) {
     constexpr bool c64 = sizeof(S) == 8;
+    if constexpr (SWEEP == kWriteGram1) {                 // plain layout with runs of 2 only
+        return vg == 1 && lr == 1 ? by_ntq<S, 2, 1, SWEEP>(ntq, route, route2) : nullptr;
+    } else
     if constexpr (SWEEP == kWriteGram || SWEEP == kWriteDirect) {
         if (vg != 2) return nullptr;
         if constexpr (c64) {
@@ -1012,6 +1028,7 @@ void* sweep_kernel_for(int storage, int lr, int vg, int ntq, int route, int swee
             case kWrite: return by_layout<float2, kWrite>(lr, vg, ntq, route, route2);
             case kWriteGram: return by_layout<float2, kWriteGram>(lr, vg, ntq, route, route2);
             case kWriteDirect: return by_layout<float2, kWriteDirect>(lr, vg, ntq, route, route2);
+            case kWriteGram1: return by_layout<float2, kWriteGram1>(lr, vg, ntq, route, route2);
         }
     } else {
         switch (sweep) {
@@ -1020,6 +1037,7 @@ void* sweep_kernel_for(int storage, int lr, int vg, int ntq, int route, int swee
             case kWrite: return by_layout<double2, kWrite>(lr, vg, ntq, route, route2);
             case kWriteGram: return by_layout<double2, kWriteGram>(lr, vg, ntq, route, route2);
             case kWriteDirect: return by_layout<double2, kWriteDirect>(lr, vg, ntq, route, route2);
+            case kWriteGram1: return by_layout<double2, kWriteGram1>(lr, vg, ntq, route, route2);
         }
     }
     return nullptr;

@@ -378,3 +378,36 @@ def test_relabelled_steps_vs_oracle(gadget_plan, tmp_path):
     dump, dnk, dntq, dls = egs.planio.read_dump(str(tmp_path / "d_0.bin"), c128=True)
     assert (dnk, dntq) == (nk, ntq) and abs(dls - ls) < 1e-12
     assert np.max(np.abs(np.asarray(dump) - tab)) < 1e-12 * max(1.0, np.max(np.abs(tab)))
+
+
+@pytest.mark.parametrize("storage", [gs.C64, gs.C128])
+def test_one_slot_folded_norms_vs_oracle(gadget_plan, tmp_path, storage):
+    """14 of the gadget instance's data steps (from step 24 on) are one-slot, one-output steps whose slot space is
+    the whole table the previous step writes: that write also sums their Gram matrix.  Hybrid samples with those folded norms and with
+    every norm pass as its own sweep equal the oracle (bits identical; log2 Pr and conditionals 1e-12 in complex128,
+    1e-5 in complex64) and each other."""
+    import subprocess
+    path, inst = gadget_plan
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    ns, na = inst.N, inst.N - inst.n
+    hy = np.full((6, ns), -1, np.int8)
+    hy[:, :na] = 0
+    egs.planio.write_strings(str(tmp_path / "hy.txt"), hy)
+    cmd = [os.path.join(root, "oracle", "gs_oracle")] + (["--c128"] if storage == gs.C128 else [])
+    ref = [l.split() for l in subprocess.run(cmd + [path, "hybrid", str(tmp_path / "hy.txt"), "23"],
+                                             capture_output=True, text=True, check=True).stdout.splitlines() if l.strip()]
+    print(f"gadget n=48 band 18 + 24 ancillas, one-slot folded norms, storage={TAG[storage]}", flush=True)
+    got = {}
+    with gs.Engine(path, storage) as e:
+        geo = [e.geometry(k) for k in range(ns)]
+        one_slot = [k for k in range(ns - 1) if geo[k]["fusable"] and geo[k + 1]["ntq"] == 1]
+        assert len(one_slot) >= 10, one_slot
+        for fusion in (True, False):
+            e.set_fusion(fusion)
+            got[fusion] = e.hybrid(hy, 23)
+    for fusion, (ob, lp, cond) in got.items():
+        assert [egs.planio.format_bits(b) for b in ob] == [r[0] for r in ref], fusion
+        assert close(lp, [float(r[1]) for r in ref], 0) <= TOL[storage]
+        assert close(cond, [[float(x) for x in r[2:]] for r in ref], 0) <= max(TOL[storage], 1e-9)
+    assert np.array_equal(got[True][0], got[False][0])
+    assert close(got[True][1], got[False][1], 0) <= TOL[storage]
