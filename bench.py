@@ -29,7 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
-SWEEP_NAMES = ("gram", "direct", "write", "write+gram", "write+direct")
+SWEEP_NAMES = ("gram", "direct", "write", "write+gram", "write+direct", "write+gram of a one-slot next step")
 ROUTES = ("m1", "split", "mixA", "mixB")
 
 
@@ -130,10 +130,12 @@ def main():
         for sw in range(5):
             if nl[k, sw] == 0:
                 continue
-            if sw >= 3:
+            if sw >= 3 and gk["fvg"] == 2:
                 key = (1 << gk["flr"], 2, gk["ntq"], gk["route"], sw, gk["route2"])   # fused layout: 2 runs
+            elif sw >= 3:
+                key = (1 << gk["flr"], 1, gk["ntq"], gk["route"], 5, 0)           # plain layout folding a 1-slot Gram
             else:
-                key = (1 << gk["lr"], 1, gk["ntq"], gk["route"], sw, 0)
+                key = (1 << gk["lr"], gk["vg"], gk["ntq"], gk["route"], sw, 0)
             a = agg.setdefault(key, dict(ms=0.0, n=0, bytes=0.0, steps=[]))
             a["ms"] += ms[k, sw]
             a["n"] += int(nl[k, sw])
@@ -145,7 +147,7 @@ def main():
     vr, vg, ntq, route, sw, route2 = key
     kname = f"gs::sweep_kernel<float2, {vr}, {vg}, {ntq}, {route}, {sw}, {route2}>"
     kdesc = (f"{vr * vg} cosets/thread ({vg} run(s) of {vr}), {ntq} slots/coset, route {ROUTES[route]}, "
-             f"{SWEEP_NAMES[sw]} sweep" + (f", next route {ROUTES[route2]}" if sw >= 3 else ""))
+             f"{SWEEP_NAMES[sw]} sweep" + (f", next route {ROUTES[route2]}" if sw in (3, 4) else ""))
     traffic_meas = profiled_traffic(kname)
     total_sweep_ms = float(ms.sum())
     print(f"dominant kernel {kname} ({kdesc}): {dom['n']} launches, {dom['ms'] / dom['n']:.4f} ms avg, {achieved:.1f} GB/s; "
@@ -246,7 +248,7 @@ def workload_c3(gs, egs, args):
     with gs.BatchEngine([path], [B], storage=gs.C64) as bt:
         t0 = time.perf_counter()
         tt = at.tune_tiles(bt, [strings[:B]], cache=cache)
-        log(f"c3: tiles tuned in {time.perf_counter() - t0:.1f} s (unroll {tt['lu']})")
+        log(f"c3: tiles tuned in {time.perf_counter() - t0:.1f} s ({tt.get('choice', 'cached')}, unroll {tt['lu']})")
         bt.use_graph(True)
         bt.score([strings[:B]], want_cond=False, want_amp=False)
         lps = []
@@ -258,7 +260,7 @@ def workload_c3(gs, egs, args):
         tb = time.perf_counter() - t0
     lpb = np.concatenate(lps)
     res["single"] = {"amplitudes_per_s": 1 / t1, "strings_timed": n1}
-    res["batched"] = {"amplitudes_per_s": N / tb, "tiles": {"lg": tt["lg"], "lu": tt["lu"]},
+    res["batched"] = {"amplitudes_per_s": N / tb, "tiles": {"lg": tt["lg"], "lu": tt["lu"], "choice": tt.get("choice")},
                       "speedup_vs_single": (N / tb) * t1,
                       "max_abs_dlog2p_vs_single": float(np.max(np.abs(lpb[:n1] - lp1)))}
     log(f"c3 batched: {N / tb:.0f} amplitudes/s")

@@ -6,7 +6,8 @@ chip fills even for one lane group) and more partial sums; large lg gives few, l
 whole step sequence once per candidate lg (per-step device times from the events the engine records after every
 step), keeps for every step index the fastest candidate, applies the choice with ``set_tile`` and stores it in a JSON
 cache keyed by the plans' content digests, the lane counts and the storage, so a later process with the same batch
-skips the measurement.
+skips the measurement.  The final choice is made on graph replays of the whole batch: the per-step choice against
+every uniform tile, each at every unroll depth.
 
 Schedule.  A mixed set of instances may not fit the device at once (an instance of width w with B lanes holds
 2^w * B entries).  ``schedules`` proposes partitions into batches under a memory budget (one batch per instance;
@@ -77,18 +78,35 @@ def tune_tiles(bt, bits, candidates=CANDIDATES, unrolls=UNROLLS, cache=None, rep
         for _ in range(repeats):
             t = np.minimum(t, step_times(bt, bits))
         times[lg] = t
-    best = [int(min(candidates, key=lambda c: times[c][k])) for k in range(bt.maxns)]
-    _apply(bt, best)
+    per_step = [int(min(candidates, key=lambda c: times[c][k])) for k in range(bt.maxns)]
+    # The per-step choice comes from runs with an event after every step; the batch is then replayed as one graph,
+    # where a choice that won a step by noise can lose.  The final choice (tiles and unroll) is the fastest graph
+    # replay of the whole step sequence among the per-step choice and every uniform tile.
+    plans = {"per_step": per_step}
+    for lg in candidates:
+        plans[f"uniform_{lg}"] = [int(lg)] * bt.maxns
+    bt.use_graph(True)
     tot = {}
-    for lu in unrolls:
-        bt.set_unroll(lu)
-        tot[lu] = min(float(np.sum(step_times(bt, bits))) for _ in range(repeats))
-    lu = min(unrolls, key=lambda u: tot[u])
+    for name, lgs in plans.items():
+        _apply(bt, lgs)
+        for lu in unrolls:
+            bt.set_unroll(lu)
+            bt.score(bits, want_cond=False, want_amp=False)          # graph capture
+            best_t = np.inf
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                bt.score(bits, want_cond=False, want_amp=False)
+                best_t = min(best_t, time.perf_counter() - t0)
+            tot[(name, lu)] = 1e3 * best_t
+    name, lu = min(tot, key=tot.get)
+    best = plans[name]
+    _apply(bt, best)
     bt.set_unroll(lu)
+    bt.use_graph(False)
     table[key] = {"lg": best, "lu": int(lu)}
     _store(cache, table)
-    return dict(lg=best, lu=int(lu), cached=False, ms_default=float(np.sum(times[12])) if 12 in times else None,
-                ms_tuned=tot[lu])
+    return dict(lg=best, lu=int(lu), cached=False, choice=name, ms_default=tot.get(("uniform_12", 0)),
+                ms_tuned=tot[(name, lu)])
 
 
 def _apply(bt, lgs):

@@ -1247,6 +1247,7 @@ int gs_step_geometry(const gs_engine* eng, int k, int* out) {
     out[8] = eng->fusable[size_t(k)] ? f.lr : -1;
     out[9] = eng->fusable[size_t(k)] ? f.lg_iter : -1;
     out[10] = s.vg;
+    out[11] = eng->fusable[size_t(k)] ? f.vg : -1;
     return 0;
 }
 

@@ -275,11 +275,12 @@ class Engine:
         return ms.reshape(self.ns, 5), n.reshape(self.ns, 5)
 
     def geometry(self, k):
-        """Launch geometry of step k: dict(lr, grid, ntq, route, fusable, fgrid, route2, lg_iter, flr, flg_iter, vg)."""
-        out = np.zeros(11, np.int32)
+        """Launch geometry of step k: dict(lr, grid, ntq, route, fusable, fgrid, route2, lg_iter, flr, flg_iter, vg,
+        fvg)."""
+        out = np.zeros(12, np.int32)
         _check(library().gs_step_geometry(self._h, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
-        return dict(zip(("lr", "grid", "ntq", "route", "fusable", "fgrid", "route2", "lg_iter", "flr", "flg_iter", "vg"),
-                        (int(x) for x in out)))
+        return dict(zip(("lr", "grid", "ntq", "route", "fusable", "fgrid", "route2", "lg_iter", "flr", "flg_iter", "vg",
+                         "fvg"), (int(x) for x in out)))
 
     def columns(self, k, fused=False):
         """Coset-coordinate columns of step k as the plain or fused sweep uses them (None: no such layout)."""

@@ -151,9 +151,10 @@ double gs_last_device_ms(const gs_engine* eng);
  * (sweeps: 0 Gram norms, 1 direct norms, 2 write, 3 write + next Gram, 4 write + next direct). */
 int gs_profile_enable(gs_engine* eng, int on);
 int gs_profile_read(const gs_engine* eng, double* ms, int64_t* launches);
-/* Launch geometry of step k, out[11]: log2 cosets per run, blocks, slots per coset, route, fusable, blocks of the
+/* Launch geometry of step k, out[12]: log2 cosets per run, blocks, slots per coset, route, fusable, blocks of the
  * fused sweep, next step's route, log2 tiles per block, log2 run of the fused layout, its log2 tiles per block
- * (those two -1 when the step does not fuse), runs per thread of the plain sweeps (1 or 2). */
+ * (those two -1 when the step does not fuse), runs per thread of the plain sweeps (1 or 2), runs per thread of the
+ * fused sweep (2: next step has two slots; 1: the plain layout folding a one-slot next step's Gram; -1: no fusion). */
 int gs_step_geometry(const gs_engine* eng, int k, int* out);
 /* Coset-coordinate columns (physical address vectors) of step k as the plain (fused = 0) or fused sweep uses them,
  * into cols[0 .. cap-1]; returns nk, or -1 when the step or layout does not exist. */
