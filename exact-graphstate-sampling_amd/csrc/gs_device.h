@@ -55,6 +55,7 @@ struct KStep {
     int eps;               // fused: the partner of output tau is output tau ^ eps of the paired coset
     int phig, nug;         // fused: slot-index and sign parity of g
     int rev;               // walk the tiles of each block in reverse Gray order
+    int xcd;               // give each XCD a contiguous eighth of the block coordinates (grids of 8 blocks or more)
     int sgg, sgk;          // kWriteGram1: next step's sign parity of g, and its constant (pmask[2]: per coordinate)
 };
 

@@ -251,6 +251,7 @@ struct gs_engine {
     int small_nk = 10;              // runs of steps with nk <= small_nk go to one single-block launch (-1: never)
     int walk = 0;                   // tile order: 0 forward, 1 reverse on odd steps, 2 reverse on every step
     int fused_run = -1;             // log2 run of the fused layout (-1: the storage's default)
+    int xcd_blocks = 1;             // XCD-contiguous block coordinates in every sweep
     int pair_ntq = 0;               // plain sweeps of steps with at most this many slots per coset take two runs per thread
     int pair_far = 0;               // ... the second one at the top coordinate (1) or the next unit bit (0)
     // optional per-sweep timing (bench / profiling)
@@ -690,7 +691,10 @@ int build_steps(gs_engine* e) {
         L.write = sweep_kernel_for(e->storage, ks.lr, ks.vg, ks.ntq, ks.route, kWrite, 0);
         if (!L.gram || !L.direct || !L.write) return fail(15, "no kernel for step " + std::to_string(k));
     }
-    for (int k = 0; k < P.ns; k++) e->hplain[size_t(k)].rev = e->walk == 2 || (e->walk == 1 && (k & 1));
+    for (int k = 0; k < P.ns; k++) {
+        e->hplain[size_t(k)].rev = e->walk == 2 || (e->walk == 1 && (k & 1));
+        e->hplain[size_t(k)].xcd = e->xcd_blocks;
+    }
     for (int k = 0; k + 1 < P.ns; k++) {
         KStep f;
         auto& L = e->launch[size_t(k)];
@@ -1311,6 +1315,22 @@ int gs_set_plain_pairs(gs_engine* eng, int ntq_max, int far) {
     HIP_TRY(hipStreamSynchronize(eng->stream));
     eng->pair_ntq = ntq_max;
     eng->pair_far = far;
+    if (int rc = build_steps(eng)) return rc;
+    const size_t ns = size_t(eng->plan.ns);
+    if (ns) {
+        HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+// Give each XCD a contiguous eighth of every sweep's block coordinates (on = 1) or deal them round-robin (0); results
+// do not depend on it beyond the order of the FP64 block sums.  Rebuilds the step descriptors.
+int gs_set_xcd_blocks(gs_engine* eng, int on) {
+    if (!eng) return fail(16, "null engine");
+    HIP_TRY(hipSetDevice(eng->device));
+    HIP_TRY(hipStreamSynchronize(eng->stream));
+    eng->xcd_blocks = on ? 1 : 0;
     if (int rc = build_steps(eng)) return rc;
     const size_t ns = size_t(eng->plan.ns);
     if (ns) {

@@ -575,8 +575,13 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     const bool active = nk >= lt || (uint64_t(tid) << LV) < (uint64_t(1) << nk);
     uint64_t a_blk = 0;
     int p_blk = 0;
+    // s.xcd: blocks are dealt round-robin to the 8 XCDs, so block i's XCD is i mod 8; rotating the 3 low bits of the
+    // block index to the top gives each XCD one contiguous eighth of the block coordinates (of each tile window)
+    const unsigned bid = s.xcd && s.lg_grid >= 3
+                             ? (blockIdx.x >> 3) | ((blockIdx.x & 7u) << (s.lg_grid - 3))
+                             : blockIdx.x;
     for (int b = 0; b < s.lg_grid; b++)
-        if ((blockIdx.x >> b) & 1) {
+        if ((bid >> b) & 1) {
             a_blk ^= s.cols[lt + b];
             p_blk ^= pb(lt + b);
         }

@@ -21,8 +21,8 @@ SEED_PLAIN, SEED_SEQ = 0, 1
 ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy", "gs_engine_create",
                "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
                "gs_last_device_ms", "gs_last_error", "gs_profile_enable", "gs_profile_read",
-               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_grid_log2_one_slot", "gs_set_small_steps", "gs_set_walk", "gs_set_plain_pairs", "gs_set_fused_run", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
-               "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_last_ms", "gs_batch_last_error",
+               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_grid_log2_one_slot", "gs_set_small_steps", "gs_set_walk", "gs_set_xcd_blocks", "gs_set_plain_pairs", "gs_set_fused_run", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
+               "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_set_xcd_items", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms", "gs_tt_plan_load", "gs_tt_plan_info",
                "gs_tt_plan_destroy", "gs_tt_engine_create", "gs_tt_engine_destroy", "gs_tt_prob", "gs_tt_chain",
                "gs_tt_sample", "gs_tt_hybrid", "gs_tt_prefix", "gs_tt_resume", "gs_tt_dp", "gs_tt_last_error")
@@ -75,6 +75,7 @@ def library():
         "gs_set_grid_log2": (I, [P, I]),
         "gs_set_small_steps": (I, [P, I]),
         "gs_set_grid_log2_one_slot": (I, [P, I]),
+        "gs_set_xcd_blocks": (I, [P, I]),
         "gs_set_walk": (I, [P, I]),
         "gs_set_plain_pairs": (I, [P, I, I]),
         "gs_set_fused_run": (I, [P, I]),
@@ -88,6 +89,7 @@ def library():
         "gs_batch_score": (I, [P, ctypes.POINTER(i8p), ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.POINTER(dp)]),
         "gs_batch_use_graph": (I, [P, I]),
         "gs_batch_set_tile": (I, [P, I, I, I]),
+        "gs_batch_set_xcd_items": (I, [P, I]),
         "gs_batch_set_unroll": (I, [P, I]),
         "gs_tt_plan_load": (I, [ctypes.c_char_p, I, pp]),
         "gs_tt_plan_info": (I, [P, ip, ip, ip]),
@@ -305,6 +307,11 @@ class Engine:
         the second at the top coordinate (far=1) or at the next unit bit (far=0)."""
         _check(library().gs_set_plain_pairs(self._h, int(ntq_max), int(far)))
 
+    def set_xcd_blocks(self, on):
+        """Each XCD takes a contiguous eighth of every sweep's block coordinates (True, default) or blocks go
+        round-robin."""
+        _check(library().gs_set_xcd_blocks(self._h, 1 if on else 0))
+
     def set_small_steps(self, nk):
         """Run consecutive steps of at most 2^nk cosets in one single-block launch (-1: never)."""
         _check(library().gs_set_small_steps(self._h, int(nk)))
@@ -360,6 +367,11 @@ class BatchEngine:
 
     def set_tile(self, i, lg, k=-1):
         self._check(library().gs_batch_set_tile(self._h, i, k, lg))
+
+    def set_xcd_items(self, on):
+        """Work-list order: 0 round-robin over the XCDs, 1 (default) each XCD takes consecutive items (lane group by
+        lane group), 2 each XCD takes consecutive coset ranges with all their lane groups."""
+        self._check(library().gs_batch_set_xcd_items(self._h, int(on)))
 
     def set_unroll(self, lu):
         """log2 cosets each lane gathers at once (0..3)."""

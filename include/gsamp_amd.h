@@ -95,6 +95,9 @@ int gs_batch_score(gs_batch* batch, const int8_t* const* bits, double* const* lo
 int gs_batch_use_graph(gs_batch* batch, int on);
 /* log2 cosets per work item of instance i at step k (k < 0: every step); not below the creation default. */
 int gs_batch_set_tile(gs_batch* batch, int i, int k, int lg);
+/* Work-list order: 0 round-robin over the XCDs, 1 (default) each XCD takes consecutive items (lane group by lane
+   group), 2 each XCD takes consecutive coset ranges with all their lane groups. */
+int gs_batch_set_xcd_items(gs_batch* batch, int on);
 /* log2 cosets each lane gathers at once (0..3; default 2 for complex64, 1 for complex128). */
 int gs_batch_set_unroll(gs_batch* batch, int lu);
 /* Device time of the last gs_batch_score (ms); step_end_ms (may be NULL) [max ns]: end of each step after the start. */
@@ -170,6 +173,9 @@ int gs_set_grid_log2_one_slot(gs_engine* eng, int lg);
 int gs_set_fused_run(gs_engine* eng, int lr);
 /* Tile order of the sweeps: 0 forward (default), 1 reverse on odd steps, 2 reverse on every step. */
 int gs_set_walk(gs_engine* eng, int mode);
+/* Each XCD takes a contiguous eighth of every sweep's block coordinates (on = 1, default) or blocks go round-robin
+   (0). */
+int gs_set_xcd_blocks(gs_engine* eng, int on);
 /* Plain sweeps of steps with at most ntq_max slots per coset (0..2; default 0, off) take two runs per thread, the
    second at the top coordinate (far = 1) or at the next unit bit (far = 0). */
 int gs_set_plain_pairs(gs_engine* eng, int ntq_max, int far);

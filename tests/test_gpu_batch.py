@@ -56,7 +56,8 @@ def test_batch_mixed_instances_match_single_engine(golden):
 
 
 def test_batch_tiles_do_not_change_results(tmp_path):
-    """n = 60 band 24 instance (width 20: 2^20 x 64 lanes = 537 MB arena): tiles and unroll depths change only the
+    """n = 60 band 24 instance (width 20: 2^20 x 64 lanes = 537 MB arena): tiles, unroll depths and the order of
+    the work lists (round-robin, or consecutive per XCD in group-major or coset-major order) change only the
     summation order of the per-lane norms."""
     inst = egs.instances.config_instance("c4")
     plan = str(tmp_path / "c4.txt")
@@ -67,10 +68,11 @@ def test_batch_tiles_do_not_change_results(tmp_path):
         print("batch c4 x64 default tiles", flush=True)
         lp, cond, amp = bt.score(bits)
         runs = []
-        for lg, lu in ((14, 2), (10, 0), (12, 1), (12, 3)):
+        for lg, lu, xcd in ((14, 2, 0), (10, 0, 1), (12, 1, 2), (12, 3, 1), (9, 1, 2)):
             bt.set_tile(0, lg)
             bt.set_unroll(lu)
-            print(f"batch c4 x64 tiles 2^{lg} unroll {lu}", flush=True)
+            bt.set_xcd_items(xcd)
+            print(f"batch c4 x64 tiles 2^{lg} unroll {lu} xcd items {xcd}", flush=True)
             runs.append(bt.score(bits))
     for lp2, cond2, amp2 in runs:
         assert np.max(np.abs(lp[0] - lp2[0])) < 1e-9

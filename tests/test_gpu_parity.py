@@ -240,23 +240,25 @@ def c4_plan(tmp_path_factory):
     return path
 
 
-@pytest.mark.parametrize("grid_log2,walk,run", [(14, 0, -1), (9, 0, -1), (9, 1, -1), (6, 2, -1), (4, 0, -1), (9, 0, 2)])
-def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2, walk, run):
+@pytest.mark.parametrize("grid_log2,walk,run,xcd", [(14, 0, -1, 0), (9, 0, -1, 1), (9, 1, -1, 0), (6, 2, -1, 1),
+                                                    (4, 0, -1, 0), (9, 0, 2, 1), (12, 0, -1, 1)])
+def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2, walk, run, xcd):
     """n=60 band-24 instance (2^20 arena, up to 2^10 blocks per sweep): samples from the FP64 arena equal the
     oracle's bits and log2 Pr (1e-12) whatever the block cap -- one tile per block with a separate reduce kernel,
     512 looping blocks with the reduce kernel, 64 or 16 looping blocks reducing in their last block -- and whatever the
-    tile order (forward, reversed on odd steps, reversed everywhere)."""
+    tile order (forward, reversed on odd steps, reversed everywhere) and whatever the block-to-XCD mapping."""
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     import subprocess
     out = subprocess.run([os.path.join(root, "oracle", "gs_oracle"), "--c128", c4_plan, "sample", "3", "777"],
                          capture_output=True, text=True, check=True).stdout.split("\n")
     ref = [l.split() for l in out if l.strip()]
-    print(f"c4 sample 3 777 grid_log2={grid_log2} walk={walk} run={run}", flush=True)
+    print(f"c4 sample 3 777 grid_log2={grid_log2} walk={walk} run={run} xcd={xcd}", flush=True)
     storage = gs.C64 if run == 2 else gs.C128        # the 4-run fused layout is a complex64 layout
     with gs.Engine(c4_plan, storage) as e:
         e.set_grid_log2(grid_log2)
         e.set_walk(walk)
         e.set_fused_run(run)
+        e.set_xcd_blocks(xcd)
         assert max(e.geometry(k)["grid"] for k in range(e.ns)) <= 1 << grid_log2
         if run == 2:
             assert any(e.geometry(k)["flr"] == 2 for k in range(e.ns))
