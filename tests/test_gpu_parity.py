@@ -246,7 +246,8 @@ def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2, walk, run, xcd):
     """n=60 band-24 instance (2^20 arena, up to 2^10 blocks per sweep): samples from the FP64 arena equal the
     oracle's bits and log2 Pr (1e-12) whatever the block cap -- one tile per block with a separate reduce kernel,
     512 looping blocks with the reduce kernel, 64 or 16 looping blocks reducing in their last block -- and whatever the
-    tile order (forward, reversed on odd steps, reversed everywhere) and whatever the block-to-XCD mapping."""
+    tile order (forward, reversed on odd steps, reversed everywhere) and whatever the block-to-XCD mapping; a second
+    run gives bitwise the same samples."""
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     import subprocess
     out = subprocess.run([os.path.join(root, "oracle", "gs_oracle"), "--c128", c4_plan, "sample", "3", "777"],
@@ -264,10 +265,12 @@ def test_c4_samples_vs_oracle_any_grid(c4_plan, grid_log2, walk, run, xcd):
             assert any(e.geometry(k)["flr"] == 2 for k in range(e.ns))
         bits, lp = e.sample(3, 777)
         lpp = e.prob(bits)
+        bits2, lp2 = e.sample(3, 777)                # every coset once per sweep, fixed-order sums: bitwise repeatable
     tol = TOL[storage]
     assert [egs.planio.format_bits(b) for b in bits] == [r[0] for r in ref]
     assert close(lp, [float(r[1]) for r in ref], 0) < tol
     assert close(lpp, lp, 0) < tol
+    assert np.array_equal(bits2, bits) and np.array_equal(lp2, lp)
 
 
 @pytest.mark.parametrize("pair_ntq,far,fusion", [(0, 0, True), (1, 1, True), (1, 0, False), (2, 1, False),
