@@ -248,6 +248,7 @@ struct gs_engine {
     bool allow_fusion = true;
     int grid_cap = kDefaultGridLog2;  // log2 of the most blocks a sweep launches
     int grid_cap1 = kDefaultGridLog2One;  // ... a plain sweep of a one-slot step
+    int grid_cap1_big = kDefaultGridLog2OneBig;  // ... of a one-slot step with at least 2^kBigStepLog2 cosets
     int small_nk = 10;              // runs of steps with nk <= small_nk go to one single-block launch (-1: never)
     int walk = 0;                   // tile order: 0 forward, 1 reverse on odd steps, 2 reverse on every step
     int fused_run = -1;             // log2 run of the fused layout (-1: the storage's default)
@@ -683,7 +684,8 @@ int build_steps(gs_engine* e) {
                 if (((ks.pmask[0] >> a) ^ (ks.pmask[0] >> b)) & 1) ks.pmask[0] ^= (uint64_t(1) << a) | (uint64_t(1) << b);
             }
         }
-        set_geometry(ks, ks.lr + (ks.vg == 2 ? 1 : 0), ks.ntq == 1 ? e->grid_cap1 : e->grid_cap);
+        set_geometry(ks, ks.lr + (ks.vg == 2 ? 1 : 0),
+                     ks.ntq != 1 ? e->grid_cap : ks.nk >= kBigStepLog2 ? e->grid_cap1_big : e->grid_cap1);
         auto& L = e->launch[size_t(k)];
         L.grid = 1 << ks.lg_grid;
         L.gram = sweep_kernel_for(e->storage, ks.lr, ks.vg, ks.ntq, ks.route, kGram, 0);
@@ -1255,16 +1257,16 @@ int gs_step_geometry(const gs_engine* eng, int k, int* out) {
     return 0;
 }
 
-// Cap the blocks of one sweep at 2^lg (1..14, default 9, and 12 for the plain sweeps of one-slot steps); tiles
-// beyond the cap loop inside each block.  Sets both caps to lg (one_slot_only: the one-slot cap alone).  Rebuilds and
-// uploads the step descriptors (synchronous).
+// Cap the blocks of one sweep at 2^lg (1..14; default 9, and for the plain sweeps of one-slot steps 12, or 14 from
+// 2^30 cosets on); tiles beyond the cap loop inside each block.  Sets every cap to lg (one_slot_only: the one-slot
+// caps alone).  Rebuilds and uploads the step descriptors (synchronous).
 static int set_grid_caps(gs_engine* eng, int lg, bool one_slot_only) {
     if (!eng) return fail(16, "null engine");
     if (lg < 1 || lg > kMaxGridLog2) return fail(16, "grid log2 must be in 1.." + std::to_string(kMaxGridLog2));
     HIP_TRY(hipSetDevice(eng->device));
     HIP_TRY(hipStreamSynchronize(eng->stream));
     if (!one_slot_only) eng->grid_cap = lg;
-    eng->grid_cap1 = lg;
+    eng->grid_cap1 = eng->grid_cap1_big = lg;
     if (int rc = build_steps(eng)) return rc;
     const size_t ns = size_t(eng->plan.ns);
     if (ns) {

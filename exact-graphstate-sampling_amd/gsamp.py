@@ -317,12 +317,13 @@ class Engine:
         _check(library().gs_set_small_steps(self._h, int(nk)))
 
     def set_grid_log2_one_slot(self, lg):
-        """Cap the plain sweeps of one-slot steps alone at 2^lg blocks (1..14, default 12)."""
+        """Cap the plain sweeps of one-slot steps alone at 2^lg blocks (1..14; default 12, or 14 from 2^30 cosets
+        on)."""
         _check(library().gs_set_grid_log2_one_slot(self._h, int(lg)))
 
     def set_grid_log2(self, lg):
-        """Cap every sweep at 2^lg blocks (1..14, default 9; one-slot plain sweeps 12); tiles beyond the cap loop
-        inside each block.  Sets the one-slot cap to lg as well."""
+        """Cap every sweep at 2^lg blocks (1..14; default 9, one-slot plain sweeps 12, or 14 from 2^30 cosets on);
+        tiles beyond the cap loop inside each block.  Sets the one-slot caps to lg as well."""
         _check(library().gs_set_grid_log2(self._h, int(lg)))
 
 

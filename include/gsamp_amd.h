@@ -164,10 +164,10 @@ int gs_step_geometry(const gs_engine* eng, int k, int* out);
 int gs_step_columns(const gs_engine* eng, int k, int fused, uint64_t* cols, int cap);
 /* Fold the next step's norm pass into the previous write (on, default) or run it as its own sweep (off). */
 int gs_set_fusion(gs_engine* eng, int on);
-/* Cap one sweep at 2^lg blocks (1..14, default 9; plain sweeps of one-slot steps 12); tiles beyond the cap loop inside
-   each block.  Sets the one-slot cap to lg as well. */
+/* Cap one sweep at 2^lg blocks (1..14; default 9, plain sweeps of one-slot steps 12, or 14 from 2^30 cosets on); tiles
+   beyond the cap loop inside each block.  Sets the one-slot caps to lg as well. */
 int gs_set_grid_log2(gs_engine* eng, int lg);
-/* Cap the plain sweeps of one-slot steps alone at 2^lg blocks (1..14, default 12). */
+/* Cap the plain sweeps of one-slot steps alone at 2^lg blocks (1..14; default 12, or 14 from 2^30 cosets on). */
 int gs_set_grid_log2_one_slot(gs_engine* eng, int lg);
 /* log2 run length of the fused layout (complex64: 1 default or 2; complex128: 0; -1 restores the default). */
 int gs_set_fused_run(gs_engine* eng, int lr);
