@@ -36,11 +36,13 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 from _pkg import egs  # noqa: E402
 
 GATE_VALUES = [-69.508530214929, -68.876165103181]   # cloud_ibmxeb/run.sh:19
+DOPED_C128 = [-68.295331132, -69.167144551]          # results/doped/run15_c128/README.md:10-13 (first, last shot)
+DOPED_C64 = [-68.295330880, -69.167143120]
 
 
 def run(binary, args, c128=False):
     exe = os.path.join(REFBIN, "gsamp2_c128" if c128 else "gsamp2")
-    res = subprocess.run([exe] + args, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="4"))
+    res = subprocess.run([exe] + args, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS=os.environ.get("GS_GOLDEN_THREADS", "4")))
     if res.returncode:
         raise RuntimeError(f"{exe} {args}: exit {res.returncode} {res.stderr}")
     return res.stdout
@@ -75,6 +77,14 @@ def doped():
     copy("data_doped/nq70_depth70_checks27_doped.qasm", os.path.join(d, "nq70_depth70_checks27_doped.qasm"))
     copy("cloud_doped/doped_m20_step_vertices.json", os.path.join(d, "doped_m20_step_vertices.json"))
     copy("cloud_doped/doped_m20_inplace_plan.txt", os.path.join(d, "doped_m20_inplace_plan.txt"))
+    # IBM's first and last doped shots (Qiskit order, cloud_doped/run_c128.sh:24: head -1 / grep . | tail -1) with the
+    # reference's published log2 Pr_doped of each in complex128 and complex64 (results/doped/run15_c128/README.md:10-13)
+    with open(os.path.join(REF, "cloud_doped/ibm_doped_shots_qiskit.txt")) as f:
+        shots = [l.strip() for l in f if l.strip()]
+    with open(os.path.join(d, "ibm_doped_shots_first_last.txt"), "w") as f:
+        f.write(shots[0] + "\n" + shots[-1] + "\n")
+    with open(os.path.join(d, "ibm_doped_shots_values.json"), "w") as f:
+        json.dump({"c128": DOPED_C128, "c64": DOPED_C64, "nd": 70, "na": 468}, f)
 
 
 def c1():
@@ -161,7 +171,7 @@ def small():
 
 def run_bin(name, args):
     exe = os.path.join(REFBIN, name)
-    res = subprocess.run([exe] + args, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS="4"))
+    res = subprocess.run([exe] + args, capture_output=True, text=True, env=dict(os.environ, OMP_NUM_THREADS=os.environ.get("GS_GOLDEN_THREADS", "4")))
     if res.returncode:
         raise RuntimeError(f"{exe} {args}: exit {res.returncode} {res.stderr}")
     return res.stdout
@@ -209,6 +219,123 @@ def twotable():
     copy("cloud_chain/plan_rev.txt", os.path.join(d, "ibm_plan_rev.txt"))
     copy("cloud_chain/gate_rev.txt", os.path.join(d, "ibm_gate_rev.txt"))
     copy("results/chain/gate_rev_out.txt", os.path.join(d, "ibm_gate_rev_chain.txt"))
+
+
+def zero_row_plan(text, k):
+    """The plan text with row b = 1 of step k's M set to zero: x_v = 1 at that step has probability exactly 0."""
+    toks = text.split()
+    ns = int(toks[1])
+    pos = 3
+    for j in range(ns):
+        nk, nt = int(toks[pos + 1]), int(toks[pos + 5])
+        pos += 6 + nk + 4 * nt
+        if j == k:
+            toks[pos + 4:pos + 8] = ["0", "0", "0", "0"]
+        pos += 8
+    lines = [" ".join(toks[:3])]
+    pos = 3
+    for j in range(ns):
+        nk, nt = int(toks[pos + 1]), int(toks[pos + 5])
+        end = pos + 6 + nk + 4 * nt + 8
+        lines.append(" ".join(toks[pos:end]))
+        pos = end
+    return "\n".join(lines) + "\n"
+
+
+def zero():
+    """Zero-probability branches (gsamp2.cpp:48, :67, :94, :116): the config-1 plan with row b = 1 of step 5's M zeroed.
+    Strings forcing b_5 = 1 have probability 0: prob and chain give -inf, chain's conditionals are 0 from step 5 on,
+    hybrid gives -inf with conditionals 0 and sampled bits 0 from step 5 on.  The reference executor's outputs in
+    complex64 and complex128."""
+    d = os.path.join(HERE, "zero")
+    os.makedirs(d, exist_ok=True)
+    plan = os.path.join(d, "plan.txt")
+    with open(os.path.join(HERE, "c1", "plan.txt")) as f:
+        open(plan, "w").write(zero_row_plan(f.read(), 5))
+    ns = 24
+    rng = np.random.default_rng(505)
+    s = rng.integers(0, 2, size=(8, ns)).astype(np.int8)
+    s[:4, 5] = 1
+    s[4:, 5] = 0
+    egs.planio.write_strings(os.path.join(d, "strings.txt"), s)
+    hy = rng.integers(0, 2, size=(6, ns)).astype(np.int8)
+    hy[:, ns - 8:] = -1
+    hy[:3, 5] = 1
+    hy[3:, 5] = 0
+    hy[5, 2:] = -1                      # a sampled step 5 (never picks the zero branch) and a sampled prefix
+    egs.planio.write_strings(os.path.join(d, "hybrid.txt"), hy)
+    for c128 in (False, True):
+        tag = "c128" if c128 else "c64"
+        for mode, args in (("prob", ["prob", os.path.join(d, "strings.txt")]),
+                           ("chain", ["chain", os.path.join(d, "strings.txt")]),
+                           ("sample", ["sample", "8", "77"]),
+                           ("hybrid", ["hybrid", os.path.join(d, "hybrid.txt"), "41"])):
+            with open(os.path.join(d, f"{mode}_{tag}.txt"), "w") as f:
+                f.write(run("gsamp2", [plan] + args, c128))
+
+
+def c4():
+    """The n = 60, band-24 instance's plan (nk up to 20: two of the oracle's 10-bit lookup chunks)
+    with the reference executor's outputs of every mode in complex64 and complex128, so that the oracle is pinned
+    at the sizes where the GPU tests use it as the checker."""
+    d = os.path.join(HERE, "c4")
+    os.makedirs(d, exist_ok=True)
+    inst = egs.instances.config_instance("c4")
+    plan = os.path.join(d, "plan.txt")
+    inst.write_plan(plan)
+    ns = len(inst.steps)
+    rng = np.random.default_rng(404)
+    egs.planio.write_strings(os.path.join(d, "strings.txt"), rng.integers(0, 2, size=(6, ns)))
+    hy = rng.integers(0, 2, size=(3, ns)).astype(np.int8)
+    hy[:, ns - 12:] = -1
+    egs.planio.write_strings(os.path.join(d, "hybrid.txt"), hy)
+    K = ns - 12
+    for c128 in (False, True):
+        tag = "c128" if c128 else "c64"
+        for mode, args in (("prob", ["prob", os.path.join(d, "strings.txt")]),
+                           ("chain", ["chain", os.path.join(d, "strings.txt")]),
+                           ("sample", ["sample", "3", "4040"]),
+                           ("sample2", ["sample2", "3", "4040"]),
+                           ("hybrid", ["hybrid", os.path.join(d, "hybrid.txt"), "44"])):
+            with open(os.path.join(d, f"{mode}_{tag}.txt"), "w") as f:
+                f.write(run("gsamp2", [plan] + args, c128))
+        dump = os.path.join(d, f"dump_{tag}")
+        with open(os.path.join(d, f"prefix_{tag}.txt"), "w") as f:
+            f.write(run("gsamp2", [plan, "prefix", os.path.join(d, "hybrid.txt"), str(K), dump], c128))
+        with open(os.path.join(d, f"resume_{tag}.txt"), "w") as f:
+            f.write(run("gsamp2", [plan, "resume", os.path.join(d, "hybrid.txt"), str(K), "45", dump], c128))
+    with open(os.path.join(d, "meta.json"), "w") as f:
+        json.dump(dict(K=K, sample_seed=4040, hybrid_seed=44, resume_seed=45, ns=ns, R=inst.R), f)
+
+
+def c2():
+    """The n = 70, band-30 sampling instance's plan (R = 30, nk up to 29: three of the oracle's 10-bit lookup chunks)
+    with the reference executor's first 3 samples from seed 4242 in complex64 and complex128."""
+    d = os.path.join(HERE, "c2")
+    os.makedirs(d, exist_ok=True)
+    plan = os.path.join(d, "plan.txt")
+    egs.instances.config_instance("c2").write_plan(plan)
+    for c128 in (False, True):
+        with open(os.path.join(d, f"sample_{'c128' if c128 else 'c64'}.txt"), "w") as f:
+            f.write(run("gsamp2", [plan, "sample", "3", "4242"], c128))
+
+
+def kats():
+    """Larger known-answer sets the reference's cloud runs produced, copied as data:
+    * ibm/uniform64.txt + uniform64_out.txt -- 64 uniform strings (cloud_ibmxeb/uniform64.txt, step order) and their
+      log2 Pr from the two-table sampler on cloud_ibmxeb/nq70_w30_plan.txt (results/ibmxeb/uniform_out.txt);
+    * ibm/samples_nq70_first4.txt -- the first 4 exact samples of the two-table sampler on cloud/nq70_plan.txt, seeds
+      1000000.. (results/samples_nq70.txt);
+    * tt/check55_plan.txt + check55_expected.txt -- the 55-qubit bit-exact gate of cloud/run.sh:19-27
+      (gsamp check55_plan.txt sample 3 1)."""
+    d = os.path.join(HERE, "ibm")
+    copy("cloud_ibmxeb/uniform64.txt", os.path.join(d, "uniform64.txt"))
+    copy("results/ibmxeb/uniform_out.txt", os.path.join(d, "uniform64_out.txt"))
+    with open(os.path.join(REF, "results/samples_nq70.txt")) as f:
+        open(os.path.join(d, "samples_nq70_first4.txt"), "w").write("".join(f.readlines()[:4]))
+    t = os.path.join(HERE, "tt")
+    copy("cloud/check55_plan.txt", os.path.join(t, "check55_plan.txt"))
+    copy("cloud/check55_expected.txt", os.path.join(t, "check55_expected.txt"))
 
 
 if __name__ == "__main__":
