@@ -3,6 +3,7 @@
 ``build()`` always recompiles.  Everything lands in ``bin/`` next to this file (git-ignored, shipped with the tree).
 The HIP sources are compiled for gfx950 only, with packed-FP32 instruction selection disabled.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -30,12 +31,16 @@ def _run(cmd, quiet_warn=True):
 
 def build_library():
     os.makedirs(BIN, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in ("gs_kernels.hip", "gs_engine.hip", "gs_batch.hip", "gs_twotable.hip", "gs_plan.cpp"):
         obj = os.path.join(BIN, src.split(".")[0] + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        _run([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:     # one hipcc process per source file
+        for f in [pool.submit(_run, c) for c in cmds]:
+            f.result()
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)

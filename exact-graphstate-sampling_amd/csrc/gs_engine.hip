@@ -818,8 +818,8 @@ int run_forced(gs_engine* e, int mode, const int8_t* bits, int64_t nstr, double*
         for (int i = 0; i < cnt; i++) {
             log2p[s0 + i] = e->h_out[i * 4];
             if (cond) memcpy(cond + (s0 + i) * ns, e->h_cond + size_t(i) * ns, size_t(ns) * sizeof(double));
-            if (amp) {
-                const std::complex<double> z = read_amp(e, i);
+            if (amp) {                  // a string of probability 0 stopped early: its amplitude is 0
+                const std::complex<double> z = e->h_out[i * 4 + 3] != 0 ? std::complex<double>() : read_amp(e, i);
                 amp[2 * (s0 + i)] = z.real();
                 amp[2 * (s0 + i) + 1] = z.imag();
             }
@@ -1259,14 +1259,16 @@ int gs_step_geometry(const gs_engine* eng, int k, int* out) {
 
 // Cap the blocks of one sweep at 2^lg (1..14; default 9, and for the plain sweeps of one-slot steps 12, or 14 from
 // 2^30 cosets on); tiles beyond the cap loop inside each block.  Sets every cap to lg (one_slot_only: the one-slot
-// caps alone).  Rebuilds and uploads the step descriptors (synchronous).
+// caps alone); lg = 0 restores the defaults of the caps it sets.  Rebuilds and uploads the step descriptors
+// (synchronous).
 static int set_grid_caps(gs_engine* eng, int lg, bool one_slot_only) {
     if (!eng) return fail(16, "null engine");
-    if (lg < 1 || lg > kMaxGridLog2) return fail(16, "grid log2 must be in 1.." + std::to_string(kMaxGridLog2));
+    if (lg < 0 || lg > kMaxGridLog2) return fail(16, "grid log2 must be in 0.." + std::to_string(kMaxGridLog2));
     HIP_TRY(hipSetDevice(eng->device));
     HIP_TRY(hipStreamSynchronize(eng->stream));
-    if (!one_slot_only) eng->grid_cap = lg;
-    eng->grid_cap1 = eng->grid_cap1_big = lg;
+    if (!one_slot_only) eng->grid_cap = lg ? lg : kDefaultGridLog2;
+    eng->grid_cap1 = lg ? lg : kDefaultGridLog2One;
+    eng->grid_cap1_big = lg ? lg : kDefaultGridLog2OneBig;
     if (int rc = build_steps(eng)) return rc;
     const size_t ns = size_t(eng->plan.ns);
     if (ns) {

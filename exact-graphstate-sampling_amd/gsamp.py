@@ -247,6 +247,22 @@ class Engine:
         """Sum the slice tables (logical order, with their logscales; optional twist), then run steps K..ns-1 with
         -1 entries sampled (``gsamp2 resume``).  Returns ``(bits, log2 Pr(full), log2 Pr(prefix), cond)``."""
         b = np.ascontiguousarray(np.asarray(bits, dtype=np.int8).reshape(-1))
+        if b.size != self.ns:
+            raise ValueError("one string of ns steps")
+        if not 0 <= K <= self.ns:
+            raise ValueError("K out of range")
+        nk, ntq = ctypes.c_int(), ctypes.c_int()        # the step-K table's shape (no steps run)
+        _check(library().gs_prefix(self._h, _i8(np.zeros(self.ns, np.int8)), K, None, 0, ctypes.byref(nk),
+                                   ctypes.byref(ntq), None, None))
+        n = (1 << nk.value) * ntq.value
+        tables = [np.asarray(t).reshape(-1) for t in tables]
+        if not tables or any(t.size != n for t in tables):
+            raise ValueError(f"every slice table must hold 2^nk * ntq = {n} entries")
+        if len(logscales) != len(tables):
+            raise ValueError("one logscale per slice table")
+        if (cshift is None) != (tqshift is None) or (cshift is not None and (len(cshift) != len(tables) or
+                                                                             len(tqshift) != len(tables))):
+            raise ValueError("cshift and tqshift: both or neither, one entry per slice table")
         tabs = [np.ascontiguousarray(np.stack([np.real(t), np.imag(t)], axis=1).reshape(-1)) for t in tables]
         ptrs = (ctypes.POINTER(ctypes.c_double) * len(tabs))(*[_f8(t) for t in tabs])
         ls = np.ascontiguousarray(np.asarray(logscales, dtype=np.float64))
@@ -318,12 +334,13 @@ class Engine:
 
     def set_grid_log2_one_slot(self, lg):
         """Cap the plain sweeps of one-slot steps alone at 2^lg blocks (1..14; default 12, or 14 from 2^30 cosets
-        on)."""
+        on; 0 restores those defaults)."""
         _check(library().gs_set_grid_log2_one_slot(self._h, int(lg)))
 
     def set_grid_log2(self, lg):
         """Cap every sweep at 2^lg blocks (1..14; default 9, one-slot plain sweeps 12, or 14 from 2^30 cosets on);
-        tiles beyond the cap loop inside each block.  Sets the one-slot caps to lg as well."""
+        tiles beyond the cap loop inside each block.  Sets the one-slot caps to lg as well; 0 restores every
+        default."""
         _check(library().gs_set_grid_log2(self._h, int(lg)))
 
 
@@ -382,6 +399,8 @@ class BatchEngine:
         """bits: list (per plan) of int8 [lanes_i, ns_i].  Returns lists of log2p [lanes_i], cond [lanes_i, ns_i],
         amp [lanes_i] complex (cond / amp entries are None when not requested)."""
         n = len(self.plans)
+        if len(bits) != n:
+            raise ValueError(f"one bit array per plan ({n}), got {len(bits)}")
         bs = [np.ascontiguousarray(np.asarray(b, dtype=np.int8)) for b in bits]
         for b, p, ln in zip(bs, self.plans, self.lanes):
             if b.shape != (ln, p.ns):
@@ -418,7 +437,12 @@ class BatchEngine:
         checkpoint segments of ``seg`` steps (0: the segment that recomputes least within ``mem_budget`` bytes of
         copies; budget 0: 70 % of the free device memory).  Returns (results, seg)."""
         n = len(self.plans)
+        if len(bits) != n:
+            raise ValueError(f"one bit array per plan ({n}), got {len(bits)}")
         bs = [np.ascontiguousarray(np.asarray(b, dtype=np.int8)) for b in bits]
+        for b, p, ln in zip(bs, self.plans, self.lanes):
+            if b.shape != (ln, p.ns):
+                raise ValueError(f"bits must be [{ln}, {p.ns}], got {b.shape}")
         lp = [np.empty(ln) for ln in self.lanes]
         am = [np.empty((ln, 2)) for ln in self.lanes]
         G = [np.empty((ln, p.ns, 2, 2)) for ln, p in zip(self.lanes, self.plans)]

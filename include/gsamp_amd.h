@@ -50,7 +50,8 @@ int gs_prob(gs_engine* eng, const int8_t* bits, int64_t nstr, double* log2p);
 
 /* chain mode (gsamp2.cpp:56-70, :218-222): one pass per step; log2p[i], cond[i][ns] = Pr(b_k | b_<k).
  * amp_reim (may be NULL) receives [nstr][2]: Z(w_b) = sum_x prod_j M_j[b_j,x_j] (-1)^{x^T A x}, the last live
- * arena entry, so that log2 Pr = log2|Z|^2 - ns (up to the global phase the graph form drops). */
+ * arena entry, so that log2 Pr = log2|Z|^2 - ns (up to the global phase the graph form drops); 0 for a string that
+ * meets a zero-probability branch. */
 int gs_chain(gs_engine* eng, const int8_t* bits, int64_t nstr, double* log2p, double* cond, double* amp_reim);
 
 /* sample / sample2 modes (gsamp2.cpp:149-161): samples first..first+count-1, each from its own generator
@@ -165,9 +166,10 @@ int gs_step_columns(const gs_engine* eng, int k, int fused, uint64_t* cols, int 
 /* Fold the next step's norm pass into the previous write (on, default) or run it as its own sweep (off). */
 int gs_set_fusion(gs_engine* eng, int on);
 /* Cap one sweep at 2^lg blocks (1..14; default 9, plain sweeps of one-slot steps 12, or 14 from 2^30 cosets on); tiles
-   beyond the cap loop inside each block.  Sets the one-slot caps to lg as well. */
+   beyond the cap loop inside each block.  Sets the one-slot caps to lg as well; lg = 0 restores all three defaults. */
 int gs_set_grid_log2(gs_engine* eng, int lg);
-/* Cap the plain sweeps of one-slot steps alone at 2^lg blocks (1..14; default 12, or 14 from 2^30 cosets on). */
+/* Cap the plain sweeps of one-slot steps alone at 2^lg blocks (1..14; default 12, or 14 from 2^30 cosets on); lg = 0
+   restores those two defaults. */
 int gs_set_grid_log2_one_slot(gs_engine* eng, int lg);
 /* log2 run length of the fused layout (complex64: 1 default or 2; complex128: 0; -1 restores the default). */
 int gs_set_fused_run(gs_engine* eng, int lr);

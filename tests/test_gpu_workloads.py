@@ -4,7 +4,8 @@
   and last doped shots scored unsliced in a complex64 arena (2^34 x 8 B = 128 GiB) and a complex128 arena (256 GiB)
   against the reference's published values (results/doped/run15_c128/README.md:10-13, scored by
   cloud_doped/run_c128.sh:24-29 with --qiskit);
-* the n = 70, band-30 sampling instance (R = 30): samples in both storages against the oracle (bits identical);
+* the n = 70, band-30 sampling instance (R = 30): samples in both storages against the reference executor's (bits
+  identical);
 * batched scoring of the n = 70, band-26 instance at 512 lanes, and one instance of each (n, band) pair of the mixed
   batch, against the oracle's chain mode;
 * the width-34 instance (n = 80, band 34, 40 gadget ancillas): complex64 against complex128 on the same seeds,
@@ -75,20 +76,16 @@ def test_doped_width34_published_shots(doped, storage, tol):
 # ---------------------------------------------------------------------------------------------------------------
 # width 30: the sampling instance
 
-@pytest.fixture(scope="module")
-def c2_plan(tmp_path_factory):
-    path = str(tmp_path_factory.mktemp("c2") / "plan.txt")
-    egs.instances.config_instance("c2").write_plan(path)
-    return path
-
-
 @pytest.mark.parametrize("storage,tol", [(gs.C64, 1e-5), (gs.C128, 1e-12)])
-def test_c2_samples_vs_oracle(c2_plan, storage, tol):
-    """3 exact samples of the n = 70, band-30 instance in a 2^30 arena: bits identical to the oracle's in the same
-    storage, log2 Pr within 1e-12 (complex128) / 1e-5 (complex64); prob of the sampled strings agrees."""
-    ref = oracle(c2_plan, "sample", 3, 4242, c128=storage == gs.C128)
-    print(f"c2 sample 3 4242 storage={'c128' if storage == gs.C128 else 'c64'}", flush=True)
-    with gs.Engine(c2_plan, storage) as e:
+def test_c2_samples_vs_reference(golden, storage, tol):
+    """3 exact samples (seed 4242) of the n = 70, band-30 instance in a 2^30 arena: bits identical to the reference
+    executor's in the same storage (tests/golden/c2), log2 Pr within 1e-12 (complex128) / 1e-5 (complex64); prob of
+    the sampled strings agrees."""
+    d = os.path.join(golden, "c2")
+    tag = "c128" if storage == gs.C128 else "c64"
+    ref = [l.split() for l in open(os.path.join(d, f"sample_{tag}.txt")) if l.strip()]
+    print(f"c2 sample 3 4242 storage={tag}", flush=True)
+    with gs.Engine(os.path.join(d, "plan.txt"), storage) as e:
         bits, lp = e.sample(3, 4242)
         lpp = e.prob(bits)
     assert [egs.planio.format_bits(b) for b in bits] == [r[0] for r in ref]
