@@ -88,3 +88,24 @@ def test_engine_without_device_fails_loudly(golden):
     with pytest.raises(gs.GsError) as ex:
         gs.Engine(os.path.join(golden, "c1", "plan.txt"))
     assert ex.value.code in (16, 17)
+
+
+def test_bindings_validate_shapes_before_any_call():
+    """Mis-sized host arrays are rejected before the C ABI reads them (the ABI takes no lengths for them):
+    BatchEngine.score/grad check one [lanes_i, ns_i] array per plan, Engine.resume the string length and K."""
+    from types import SimpleNamespace
+    import numpy as np
+    bt = gs.BatchEngine.__new__(gs.BatchEngine)
+    bt.plans, bt.lanes, bt._h = [SimpleNamespace(ns=5), SimpleNamespace(ns=7)], [64, 128], None
+    good = [np.zeros((64, 5), np.int8), np.zeros((128, 7), np.int8)]
+    for bad in ([good[0]], [good[0], np.zeros((7, 128), np.int8)], [np.zeros((64, 4), np.int8), good[1]]):
+        with pytest.raises(ValueError):
+            bt.grad(bad)
+        with pytest.raises(ValueError):
+            bt.score(bad)
+    e = gs.Engine.__new__(gs.Engine)
+    e.ns, e._h = 6, None
+    with pytest.raises(ValueError):
+        e.resume(np.zeros(5, np.int8), 2, [np.zeros(4)], [0.0], 1)
+    with pytest.raises(ValueError):
+        e.resume(np.zeros(6, np.int8), 7, [np.zeros(4)], [0.0], 1)
