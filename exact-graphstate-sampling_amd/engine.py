@@ -12,11 +12,10 @@
 
 The reference's ``run_dp`` (engine.py:47-53) runs a binary inside a container; it has no counterpart here.
 """
-import itertools
-
 import numpy as np
 
 from .plan import plan, width
+from .sliced import assignments, fold_counts
 
 
 def _pivmask(s):
@@ -65,22 +64,17 @@ class Sliced:
         self.steps = plan(sub, list(range(self.m)))
 
     def slices(self, w):
-        """Yield ``(prefactor, weights of the kept vertices)`` for every assignment of the sliced vertices."""
-        A, S = self.A, self.S
-        for xs in itertools.product((0, 1), repeat=len(S)):
+        """Yield ``(prefactor, weights of the kept vertices)`` for every assignment of the sliced vertices: the
+        prefactor is prod_{v in S} w_v(x_v) times the sign of the edges inside S with both ends 1, and a kept vertex
+        with an odd number of sliced neighbours set to 1 has w_q(1) negated (the edge phases (-1)^{x_v x_q})."""
+        for xs in assignments(self.S):
+            neg, inner = fold_counts(self.A, self.S, xs)
             pre = 1 + 0j
-            for v, xv in zip(S, xs):
+            for v, xv in zip(self.S, xs):
                 pre *= w[v][xv]
-            inner = 0
-            for a in range(len(S)):
-                for b in range(a + 1, len(S)):
-                    inner ^= int(A[S[a], S[b]]) & xs[a] & xs[b]
-            pre *= (-1) ** inner
-            ww = []
-            for q in self.keep:
-                flip = sum(int(A[q, v]) & xv for v, xv in zip(S, xs)) & 1
-                ww.append((w[q][0], -w[q][1] if flip else w[q][1]))
-            yield pre, ww
+            if sum(inner) & 1:
+                pre = -pre
+            yield pre, [(w[q][0], -w[q][1] if neg.get(q, 0) & 1 else w[q][1]) for q in self.keep]
 
 
 def write_jobs(path, sl, ws):

@@ -130,7 +130,7 @@ def plan_inplace(A, order, R, keep_order="physical"):
             raise AssertionError("new basis does not span the field space")
         Tq = {qv: apply_linear(T, qv) for qv in Qall}
         steps.append(dict(v=v, r=r, rn=len(newB), keep=keep, Tkeep=[T[i] for i in keep], rho=rho, terms=terms,
-                          Tq=Tq, m=m, g=g, B=list(B), Qall=Qall))
+                          Tq=Tq, m=m, g=g, B=list(B), Qall=Qall, Qb=list(Qb)))
         B, T = newB, newT
     if B:
         raise AssertionError("state space must be empty at the end")

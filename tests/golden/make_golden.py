@@ -320,6 +320,31 @@ def c2():
             f.write(run("gsamp2", [plan, "sample", "3", "4242"], c128))
 
 
+def sliced():
+    """The reference's committed sliced plans, as SHA-256 digests (the files are 22-195 KB each), with the small
+    twist files and the twist gate's proposals copied as data:
+    * cloud_nq80/tw70_s0.txt, tw70_s1.txt, tw70_twist.txt: IBM #151 (width-30 order) sliced at vertex 21, K = 50;
+      tw70_props.txt holds the 3 hybrid proposals of the gate cloud_nq80/run_hybrid.sh:32-43 (seed 17);
+    * cloud_doped/doped_m20sl_s00..s11.txt: the 538-vertex doped instance sliced at ancillas 135 and 104, K = 518,
+      R = 33 (cloud_doped/doped_m20sl_slices.json)."""
+    import hashlib
+    d = os.path.join(HERE, "sliced")
+    os.makedirs(d, exist_ok=True)
+    digests = {}
+    for name in ["cloud_nq80/tw70_s0.txt", "cloud_nq80/tw70_s1.txt", "cloud_nq80/tw70_twist.txt"] + \
+            [f"cloud_doped/doped_m20sl_s{t}.txt" for t in ("00", "01", "10", "11")]:
+        with open(os.path.join(REF, name), "rb") as f:
+            digests[name] = hashlib.sha256(f.read()).hexdigest()
+    meta = json.load(open(os.path.join(REF, "cloud_doped/doped_m20sl_slices.json")))
+    with open(os.path.join(d, "reference_sliced_plans.json"), "w") as f:
+        json.dump({"sha256": digests,
+                   "tw70": {"slices": [21], "K": 50, "hybrid_seed": 17},
+                   "doped_m20sl": {"slices": meta["slices"], "K": meta["K_suffix_start"], "R": meta["R"]}}, f,
+                  indent=1)
+    copy("cloud_nq80/tw70_twist.txt", os.path.join(d, "tw70_twist.txt"))
+    copy("cloud_nq80/tw70_props.txt", os.path.join(d, "tw70_props.txt"))
+
+
 def kats():
     """Larger known-answer sets the reference's cloud runs produced, copied as data:
     * ibm/uniform64.txt + uniform64_out.txt -- 64 uniform strings (cloud_ibmxeb/uniform64.txt, step order) and their

@@ -63,3 +63,83 @@ def test_sliced_sum_equals_unsliced(tmp_path, S, K):
     extra = ["--twist", out + "_twist.txt"] if info["twist"] else []
     got = [float(r[1]) for r in run(info["plans"][0], "resume", strings, K, 5, *dumps, *extra)]
     assert np.max(np.abs(np.array(got) - np.array(want))) < 1e-10, (got, want)
+
+
+def _digest(path):
+    import hashlib
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+@pytest.fixture(scope="module")
+def ref_sliced(golden):
+    import json
+    d = os.path.join(golden, "sliced")
+    return d, json.load(open(os.path.join(d, "reference_sliced_plans.json")))
+
+
+def test_tw70_sliced_plans_identical_to_reference(golden, ref_sliced, tmp_path):
+    """IBM #151 (cloud_nq76/nq70.qasm, width-30 order) sliced at vertex 21 with K = 50: both slice plans and the twist
+    file are byte-identical to the reference's committed cloud_nq80/tw70_s0.txt, tw70_s1.txt, tw70_twist.txt (the
+    default, physical keep order; the logical order gives other files)."""
+    import json
+    d, ref = ref_sliced
+    ibm = os.path.join(golden, "ibm")
+    n, T, tail = egs.tableau.clifford_body(os.path.join(ibm, "nq70.qasm"))
+    order = json.load(open(os.path.join(ibm, "order_w30.json")))
+    am = egs.amp.Amplitudes(n, T, tail, order)
+    out = str(tmp_path / "tw70")
+    info = sl.make_sliced_plans(am.A, am.M, order, ref["tw70"]["slices"], out, K=ref["tw70"]["K"])
+    assert info["twist"] and info["R"] == 30
+    for t in ("0", "1"):
+        assert _digest(f"{out}_s{t}.txt") == ref["sha256"][f"cloud_nq80/tw70_s{t}.txt"], t
+    assert _digest(f"{out}_twist.txt") == ref["sha256"]["cloud_nq80/tw70_twist.txt"]
+    assert open(f"{out}_twist.txt").read() == open(os.path.join(d, "tw70_twist.txt")).read()
+    out2 = str(tmp_path / "tw70l")
+    sl.make_sliced_plans(am.A, am.M, order, [21], out2, K=50, keep_order="logical")
+    assert _digest(f"{out2}_s0.txt") != ref["sha256"]["cloud_nq80/tw70_s0.txt"]
+
+
+def test_doped_sliced_plans_identical_to_reference(golden, ref_sliced, tmp_path):
+    """The 538-vertex doped instance (T-gadget conversion of data_doped/nq70_depth70_checks27_doped.qasm, the width-34
+    step order) sliced at ancillas 135 and 104: R = 33, K = 518 (the first step of the all-data suffix) and the four
+    slice plans byte-identical to the reference's committed cloud_doped/doped_m20sl_s00..s11.txt."""
+    import json
+    _, ref = ref_sliced
+    dd = os.path.join(golden, "doped")
+    nd, ops = egs.tableau.parse(os.path.join(dd, "nq70_depth70_checks27_doped.qasm"))
+    N, body, tail, anc = egs.gadgets.convert(nd, ops)
+    q = tmp_path / "doped.qasm"
+    egs.gadgets.write_qasm(str(q), N, body + tail)
+    NN, TT, tl = egs.tableau.clifford_body(str(q))
+    sv = json.load(open(os.path.join(dd, "doped_m20_step_vertices.json")))
+    am = egs.amp.Amplitudes(NN, TT, tl, sv)
+    out = str(tmp_path / "m20sl")
+    info = sl.make_sliced_plans(am.A, am.M, sv, ref["doped_m20sl"]["slices"], out, nd=nd)
+    assert (info["R"], info["K"], info["twist"]) == (ref["doped_m20sl"]["R"], ref["doped_m20sl"]["K"], False)
+    for t in ("00", "01", "10", "11"):
+        assert _digest(f"{out}_s{t}.txt") == ref["sha256"][f"cloud_doped/doped_m20sl_s{t}.txt"], t
+
+
+def test_twist_line_reads_the_q_part_off_the_pivots():
+    """twist_line splits a logical translation into kept coordinates and Q through the fully reduced Q basis; checked
+    against a search over every element of Q on random steps of random plans (the Q part is the one element that
+    agrees with the translation on every non-kept coordinate)."""
+    rng = np.random.default_rng(11)
+    checked = 0
+    for seed in range(6):
+        inst = egs.instances.Instance(14, 6, seed)
+        for s in inst.steps:
+            if not s["Qb"]:
+                continue
+            keepmask = sum(1 << i for i in s["keep"])
+            for _ in range(4):
+                a = int(rng.integers(0, 1 << s["r"]))
+                cs, ts = sl.twist_line(s, a)
+                aq = [q for q in s["Qall"] if (a ^ q) & ~keepmask == 0]
+                assert len(aq) == 1
+                ak = a ^ aq[0]
+                assert ts == s["Tq"][aq[0]]
+                assert cs == sum(((ak >> i) & 1) << j for j, i in enumerate(s["keep"]))
+                checked += 1
+    assert checked > 50

@@ -98,3 +98,22 @@ def test_resume_of_two_slices_is_linear(tdir, golden):
     whole = tt.resume(steps, bits, K, [rtab], [rls], c64=False)
     split = tt.resume(steps, bits, K, [part * 2.0 ** -3, (rtab - part) * 2.0 ** 1], [rls + 3, rls - 1], c64=False)
     assert abs(whole[0] - split[0]) < 1e-9 and np.max(np.abs(whole[2] - split[2])) < 1e-9
+
+
+def test_sliced_jobs_reproduce_the_committed_jobs(tdir, golden):
+    """engine.Sliced + write_jobs rebuild the committed dp job file (vertices 3 and 11 of the config-1 instance sliced,
+    weights of six strings) on which the reference dp binary's outputs were taken: every number equal."""
+    inst = egs.instances.Instance(24, 10, 1)
+    sl = egs.engine.Sliced(inst.am.A, inst.order, [3, 11])
+    bits = egs.planio.read_strings(os.path.join(golden, "c1", "strings.txt"), len(inst.steps))[:6]
+    ws = []
+    for b in bits:
+        bq = egs.planio.step_to_qubit(b, inst.step_vertices, inst.N)
+        ws.append([(complex(inst.am.M[q][bq[q], 0]), complex(inst.am.M[q][bq[q], 1])) for q in range(inst.N)])
+    import tempfile
+    with tempfile.TemporaryDirectory() as t:
+        path = os.path.join(t, "jobs.txt")
+        egs.engine.write_jobs(path, sl, ws)
+        got = [float(x) for x in open(path).read().split()]
+    want = [float(x) for x in open(os.path.join(tdir, "dp_jobs.txt")).read().split()]
+    assert got == want
