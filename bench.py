@@ -54,7 +54,6 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=32, help="samples in the profiled (per-kernel timing) pass")
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-samples", type=int, default=8, help="samples in the bounded CPU baseline run")
     ap.add_argument("--c128-steps", type=int, default=256)
     ap.add_argument("--ibm-steps", type=int, default=8, help="samples of the IBM width-30 plan (0: skip)")
     ap.add_argument("--tt-steps", type=int, default=2, help="samples of the two-table width-31 plan (0: skip)")
@@ -68,6 +67,9 @@ def main():
     ap.add_argument("--c5-c128", type=int, default=4, help="c5 samples/strings re-scored in a complex128 arena")
     ap.add_argument("--no-fusion", action="store_true", help="run every norm pass as its own sweep")
     ap.add_argument("--grid-log2", type=int, default=0, help="cap sweeps at 2^N blocks (default: engine default)")
+    ap.add_argument("--legs", default="c3,c4,c5",
+                    help="short legs of the other workloads appended to the headline line (comma list; '' for none)")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="repetitions of each CPU baseline timing (median)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="first run one plain device copy of this many MiB (byte-counter calibration)")
     args = ap.parse_args()
@@ -169,9 +171,16 @@ def main():
     ibm = ibm_w30(gs, egs, args.ibm_steps, args) if args.ibm_steps else None
     tt2 = ibm_two_table(gs, egs, args.tt_steps) if args.tt_steps else None
 
-    cpu = None
+    legs = {}
+    for name in [x for x in args.legs.split(",") if x]:
+        t0 = time.perf_counter()
+        legs[name] = {"c3": leg_c3, "c4": leg_c4, "c5": leg_c5}[name](gs, egs)
+        legs[name]["leg_wall_s"] = time.perf_counter() - t0
+        log(f"leg {name}: {json.dumps(legs[name])}")
+
+    cpu = cpu_detail = None
     if not args.no_cpu:
-        cpu = cpu_baseline(plan_path, args.seed, args.cpu_samples)
+        cpu, cpu_detail = cpu_baseline(plan_path, args.seed, args.cpu_reps)
 
     out = {
         "metric": "s/exact sample (n=70 band-30 banded graph state, 2^30 in-place arena, complex64 + FP64 sums)",
@@ -208,9 +217,14 @@ def main():
                         "method": "the same seeds sampled in a complex128 arena (bits compared) and the complex64 "
                                   "samples re-scored there (prob mode)"},
         "cpu_baseline": cpu,
+        "cpu_baseline_detail": cpu_detail,
         "ibm_w30": ibm,
         "ibm_two_table": tt2,
     }
+    if "c4" in legs:                          # the metric's third part: loss gradients per second (c64)
+        out["grads_per_s"] = legs["c4"]["c64"]["grads_per_s"]
+        out["string_grads_per_s"] = legs["c4"]["c64"]["string_grads_per_s"]
+    out.update(legs)
     print(json.dumps(out))
 
 
@@ -382,6 +396,133 @@ def workload_c5(gs, egs, args):
     return res
 
 
+def leg_c4(gs, egs):
+    """Gradients, short form of the c4 workload: n = 60, band 24 (seed 4), 120 angles, loss over 1024 strings; 3 timed
+    forward passes and 3 timed gradients per storage, plus a complex64 gradient with checkpoints every 10 steps, which
+    recomputes the table inside every segment (the default complex64 budget keeps every step's input)."""
+    inst = egs.instances.config_instance("c4")
+    model = egs.adjoint.AngleModel(inst)
+    path = os.path.join(tempfile.mkdtemp(prefix="gs_c4_"), "c4.txt")
+    inst.write_plan(path)
+    B, reps = 1024, 3
+    bits = np.random.default_rng(4).integers(0, 2, size=(B, len(inst.steps))).astype(np.int8)
+    res = {"workload": "c4 gradients, n=60 band 24 (seed 4), 120 angles (fp32), loss = mean -ln Pr over 1024 strings",
+           "width": inst.width}
+    for storage, tag in ((gs.C64, "c64"), (gs.C128, "c128")):
+        with gs.BatchEngine([path], [B], storage=storage) as bt:
+            bt.use_graph(True)
+            M, dA, dB = model.matrices(model.alpha0, model.beta0)
+            bt.set_matrices(0, M)
+            bt.score([bits], want_cond=False, want_amp=False)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                lp, _, _ = bt.score([bits], want_cond=False, want_amp=False)
+            fwd_wall = (time.perf_counter() - t0) / reps
+            fwd_dev = bt.last_ms()[0]
+            runs = [("", 0)] + ([("recompute_seg10_", 10)] if storage == gs.C64 else [])
+            for prefix, seg in runs:
+                bt.grad([bits], seg=seg)
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    (r,), used = bt.grad([bits], seg=seg)
+                grad_wall = (time.perf_counter() - t0) / reps
+                tot_dev, gfwd_dev = bt.grad_ms()
+                res.setdefault(tag, {}).update({
+                    prefix + "grad_ms": 1e3 * grad_wall, prefix + "grad_device_ms": tot_dev,
+                    prefix + "grad_over_forward": tot_dev / fwd_dev,
+                    prefix + "backward_over_forward": (tot_dev - gfwd_dev) / fwd_dev,
+                    prefix + "grads_per_s": 1.0 / grad_wall, prefix + "string_grads_per_s": B / grad_wall,
+                    prefix + "checkpoint_segment": used})
+            res[tag].update({"loss": float(-np.mean(lp[0]) * np.log(2)), "forward_ms": 1e3 * fwd_wall,
+                             "forward_device_ms": fwd_dev})
+        log(f"c4 {tag}: {res[tag]}")
+    return res
+
+
+def leg_c3(gs, egs):
+    """Batched scoring, short form of the c3 workload: 1024 strings (512 uniform, 512 exact samples) of the n = 70,
+    band-26 instance in two 512-lane batches (tiles tuned over 2^10 / 2^12 cosets and unroll 1 / 2, graph replays)
+    against one string per launch sequence; and a reduced mix of 12 instances (4 of each (n, band) pair of the full
+    mix, 64 lanes each) through the schedule tuner, with p50 / p99 per-instance latency."""
+    at = egs.autotune
+    tmp = tempfile.mkdtemp(prefix="gs_c3_")
+    inst = egs.instances.config_instance("c3")
+    path = os.path.join(tmp, "c3.txt")
+    inst.write_plan(path)
+    ns, N, L = len(inst.steps), 1024, 512
+    strings = np.random.default_rng(3).integers(0, 2, size=(N, ns)).astype(np.int8)
+    res = {"workload": "c3 batched scoring, n=70 band 26 (seed 3) + a 12-instance mix", "width": inst.width}
+    with gs.Engine(path, gs.C64) as e:
+        strings[N // 2:], _ = e.sample(N - N // 2, 33)
+        e.chain(strings[:4])
+        t0 = time.perf_counter()
+        lp1, _, _ = e.chain(strings[:128])
+        t1 = (time.perf_counter() - t0) / 128
+    with gs.BatchEngine([path], [L], storage=gs.C64) as bt:
+        tt = at.tune_tiles(bt, [strings[:L]], candidates=(10, 12), unrolls=(1, 2))
+        bt.use_graph(True)
+        bt.score([strings[:L]], want_cond=False, want_amp=False)
+        lps = []
+        t0 = time.perf_counter()
+        for i in range(0, N, L):
+            lp, _, _ = bt.score([strings[i:i + L]], want_cond=False, want_amp=False)
+            lps.append(lp[0])
+        tb = time.perf_counter() - t0
+    lpb = np.concatenate(lps)
+    res["single_amplitudes_per_s"] = 1 / t1
+    res["batched_amplitudes_per_s"] = N / tb
+    res["batched_over_single"] = (N / tb) * t1
+    res["tiles"] = {"lg": tt["lg"], "lu": tt["lu"], "choice": tt.get("choice")}
+    res["max_abs_dlog2p_batched_vs_single"] = float(np.max(np.abs(lpb[:128] - lp1)))
+    plans, widths, bits = [], [], []
+    for j, (n, band) in enumerate([(40, 16)] * 4 + [(56, 22)] * 4 + [(70, 26)] * 4):
+        seed = 300 + 32 * (j // 4) + j % 4                      # the first four seeds of each pair of the full mix
+        mi = egs.instances.Instance(n, band, seed)
+        pth = os.path.join(tmp, f"m{j}.txt")
+        mi.write_plan(pth)
+        plans.append(gs.Plan(path=pth))
+        widths.append(mi.R)
+        bits.append(np.random.default_rng(1000 + seed - 300).integers(0, 2, size=(64, len(mi.steps))).astype(np.int8))
+    best, sched, _, _ = at.tune_schedule(gs, plans, widths, 64, bits, 120e9, gs.C64)
+    res["mix"] = {"instances": len(plans), "lanes": 64, "widths": [min(widths), max(widths)], "best_schedule": best,
+                  "amplitudes_per_s": sched[best]["amplitudes_per_s"], "p50_s": sched[best]["p50_s"],
+                  "p99_s": sched[best]["p99_s"], "schedules": sched}
+    return res
+
+
+def leg_c5(gs, egs):
+    """Width 34, short form of the c5 workload: n = 80, band 34 (seed 5) + 40 phase-gadget ancillas in a complex64
+    arena of 2^34 entries (137 GB): 2 exact hybrid samples (ancillas forced 0) and chain-mode log2 Pr of 2 random
+    strings; the 2 samples re-scored in a complex128 arena (275 GB) give |d log2 Pr|."""
+    inst = egs.instances.config_instance("c5")
+    path = os.path.join(tempfile.mkdtemp(prefix="gs_c5_"), "c5.txt")
+    inst.write_plan(path)
+    _, R, steps = egs.planio.read_inplace_plan(path)
+    traffic = egs.planio.plan_traffic(steps, 8)
+    ns, na = len(steps), inst.N - inst.n
+    hy = np.full((2, ns), -1, np.int8)
+    hy[:, :na] = 0
+    rnd = np.random.default_rng(55).integers(0, 2, size=(2, ns)).astype(np.int8)
+    rnd[:, :na] = 0
+    with gs.Engine(path, gs.C64) as e:
+        t0 = time.perf_counter()
+        ob, lps, _ = e.hybrid(hy, 5)
+        ts = (time.perf_counter() - t0) / 2
+        t0 = time.perf_counter()
+        e.chain(rnd)
+        tc = (time.perf_counter() - t0) / 2
+    res = {"workload": "c5 width 34: n=80 band 34 seed 5 + 40 gadget ancillas, R=34, complex64", "R": R,
+           "s_per_sample": ts, "hbm_gbps_algorithmic": traffic["sample"] / ts / 1e9, "s_per_string_chain": tc,
+           "chain_gbps_algorithmic": traffic["chain1"] / tc / 1e9, "samples": 2, "strings": 2}
+    try:
+        with gs.Engine(path, gs.C128) as e2:
+            lp2 = e2.prob(ob)
+        res["max_abs_dlog2p_vs_c128"] = float(np.max(np.abs(lp2 - lps)))
+    except gs.GsError as ex:
+        res["c128_error"] = str(ex)
+    return res
+
+
 IBM_PLAN = os.path.join("tests", "golden", "ibm", "nq70_inplace_plan.txt")
 IBM_PUBLISHED_S = 11.3      # README.md:24: 1,012 samples in 190 min, gsamp2 sample2, Ryzen 9 9955HX laptop
 
@@ -452,32 +593,80 @@ def profiled_traffic(kname):
     return None
 
 
-def cpu_baseline(plan_path, seed, count):
-    """``count`` exact samples of the same plan on the host cores (one process, plan parsed once); the reference
-    executor if it was built, else the port."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "gsamp2")
-    port = os.path.join(ROOT, "oracle", "gs_oracle")
-    exe, kind = (ref, "reference") if os.path.exists(ref) else (port, "port")
-    if not os.path.exists(exe):
-        return None
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
-    t0 = time.perf_counter()
-    res = subprocess.run([exe, plan_path, "sample", str(count), str(seed)], capture_output=True, text=True, env=env,
-                         timeout=600)
-    wall = time.perf_counter() - t0
-    if res.returncode:
-        return {"error": f"exit {res.returncode}"}
-    cpu_model = ""
+def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
-            cpu_model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+            return next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
     except (OSError, StopIteration):
-        pass
-    return {"value": wall / count, "unit": "s/sample", "cores": cores, "kind": kind,
-            "sample": "%d exact samples (seeds %d..%d) of the same plan, whole process wall time incl. plan parse, "
-                      "divided by %d" % (count, seed, seed + count - 1, count),
-            "cpu": cpu_model, "line": res.stdout.strip()[:200]}
+        return ""
+
+
+def _ref_sample_s(exe, plan, seed, reps, env):
+    """Seconds per exact sample of the reference executor, excluding plan parsing, arena allocation and first touch:
+    the wall time of ``sample 2`` minus that of ``sample 1`` (both from ``seed``), median of ``reps`` pairs."""
+    diffs = []
+    for _ in range(reps):
+        t = []
+        for n in (1, 2):
+            t0 = time.perf_counter()
+            subprocess.run([exe, plan, "sample", str(n), str(seed)], capture_output=True, text=True, env=env,
+                           check=True, timeout=600)
+            t.append(time.perf_counter() - t0)
+        diffs.append(t[1] - t[0])
+    return float(np.median(diffs)), [float(x) for x in diffs]
+
+
+def _port_sample_s(exe, plan, seed, n, env, c128):
+    """Seconds per exact sample of this project's CPU restatement (oracle/gs_oracle), from its own per-line timing
+    (GS_ORACLE_TIMING: compute only, after parsing and allocation), median over n samples."""
+    cmd = [exe] + (["--c128"] if c128 else []) + [plan, "sample", str(n), str(seed)]
+    res = subprocess.run(cmd, capture_output=True, text=True, env=dict(env, GS_ORACLE_TIMING="1"), check=True,
+                         timeout=600)
+    t = [float(l.split()[1]) for l in res.stderr.splitlines() if l.startswith("time ")]
+    return float(np.median(t)), t
+
+
+def cpu_baseline(plan_path, seed, reps):
+    """The same workload on the host cores.  Headline: the reference executor compiled from its own sources
+    (oracle/_ref/gsamp2, complex64 arena), seconds per exact sample excluding parse, allocation and first touch,
+    median of ``reps`` pairs.  Detail: the same for its complex128 build and for the n = 24 config-1 plan, and this
+    project's restatement (oracle/gs_oracle) in both storages, with the restatement/reference ratios."""
+    ref64 = os.path.join(ROOT, "oracle", "_ref", "gsamp2")
+    ref128 = os.path.join(ROOT, "oracle", "_ref", "gsamp2_c128")
+    port = os.path.join(ROOT, "oracle", "gs_oracle")
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+    c1 = os.path.join(ROOT, "tests", "golden", "c1", "plan.txt")
+    detail = {"cores": cores, "cpu": _cpu_model(), "method": "reference: wall(sample 2) - wall(sample 1), median of "
+              f"{reps} pairs; port: its per-sample compute time, median of 3 samples"}
+    head = None
+    try:
+        for tag, plan in (("c2", plan_path), ("c1", c1)):
+            for storage, exe in (("c64", ref64), ("c128", ref128)):
+                if os.path.exists(exe):
+                    v, all_ = _ref_sample_s(exe, plan, seed, reps, env)
+                    detail[f"reference_{tag}_{storage}_s"] = v
+                    detail[f"reference_{tag}_{storage}_pairs_s"] = all_
+                if os.path.exists(port):
+                    v, all_ = _port_sample_s(port, plan, seed, 3, env, storage == "c128")
+                    detail[f"port_{tag}_{storage}_s"] = v
+                    if f"reference_{tag}_{storage}_s" in detail:
+                        detail[f"port_over_reference_{tag}_{storage}"] = v / detail[f"reference_{tag}_{storage}_s"]
+                log(f"cpu baseline {tag} {storage}: " + ", ".join(f"{k}={v}" for k, v in detail.items()
+                                                                    if k.endswith(f"{tag}_{storage}_s")))
+    except (subprocess.SubprocessError, OSError) as ex:
+        detail["error"] = str(ex)
+    if "reference_c2_c64_s" in detail:
+        head = {"value": detail["reference_c2_c64_s"], "unit": "s/sample", "cores": cores, "kind": "reference",
+                "sample": f"exact samples of the same plan (seed {seed}) by the reference executor built from its "
+                          f"sources, complex64 arena: wall(sample 2) - wall(sample 1), median of {reps} pairs "
+                          "(excludes plan parsing, allocation and first touch)",
+                "cpu": detail["cpu"]}
+    elif "port_c2_c64_s" in detail:
+        head = {"value": detail["port_c2_c64_s"], "unit": "s/sample", "cores": cores, "kind": "port",
+                "sample": "3 exact samples of the same plan by oracle/gs_oracle, per-sample compute time, median",
+                "cpu": detail["cpu"]}
+    return head, detail
 
 
 if __name__ == "__main__":

@@ -2,7 +2,11 @@
 
 For an instance whose qubit q ends with the tail ``rz(beta_q) h rz(alpha_q)`` (``instances.py``), the measurement
 matrix of q is M_q = W_q U_q^dagger with W_q = RZ(alpha_q) H RZ(beta_q) (amp.py:26 with the tail product of
-amp.py:16) and U_q from the graph form of the Clifford body.  The engine's adjoint returns, per string and step k,
+amp.py:16) and U_q from the graph form of the Clifford body.  RZ(alpha_q) is then the last gate before the
+measurement in the Z basis: it multiplies each outcome's amplitude by a phase, so d ln Pr / d alpha_q = 0 for every
+string, and only the 60 beta angles of the n = 60 workload carry a gradient.  ``AngleModel(inst, last_h=True)``
+models the tail ``rz(beta_q) h rz(alpha_q) h`` instead (W_q = H RZ(alpha_q) H RZ(beta_q): alpha acts as an X rotation
+before the measurement), where both angles are live.  The engine's adjoint returns, per string and step k,
 G_k(x) = dZ/dM_k[b_k, x] (``gs_batch_grad``); with Pr(b) proportional to |Z|^2,
 
     d ln Pr(b) / d theta = 2 Re( sum_x dM_q[b_q, x]/d theta * G_{k(q)}(x) / Z ),
@@ -33,17 +37,21 @@ def _drz(t):
 
 class AngleModel:
     """Measurement matrices of an ``instances.Instance`` as functions of its tail angles (alpha, beta per data
-    qubit; ancilla tails, if any, are fixed)."""
+    qubit; ancilla tails, if any, are fixed).  The angles are float32 when the instance's are (``fp32_angles``);
+    the matrices and derivatives are computed in float64 from those values.  ``last_h``: the tail is followed by
+    one more ``h`` (see the module docstring)."""
 
-    def __init__(self, inst):
+    def __init__(self, inst, last_h=False):
         self.inst = inst
+        self.last_h = last_h
         _, U = graph_form(_tableau(inst))
         self.Udag = [u.conj().T for u in U]
         self.n = inst.n
         self.N = inst.N
         self.step_vertices = inst.step_vertices
-        self.alpha0 = np.zeros(self.n)
-        self.beta0 = np.zeros(self.n)
+        dtype = np.float32 if getattr(inst, "fp32_angles", False) else np.float64
+        self.alpha0 = np.zeros(self.n, dtype)
+        self.beta0 = np.zeros(self.n, dtype)
         per = {}
         for name, qs in inst.tail:
             per.setdefault(qs[0], []).append(name)
@@ -67,6 +75,8 @@ class AngleModel:
             M[k] = _rz(a) @ _H @ _rz(b) @ self.Udag[q]
             dA[k] = _drz(a) @ _H @ _rz(b) @ self.Udag[q]
             dB[k] = _rz(a) @ _H @ _drz(b) @ self.Udag[q]
+            if self.last_h:
+                M[k], dA[k], dB[k] = _H @ M[k], _H @ dA[k], _H @ dB[k]
         return M, dA, dB
 
     def grad_ln_pr(self, bits, Z, G, dA, dB):

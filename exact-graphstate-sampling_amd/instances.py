@@ -56,8 +56,10 @@ def clifford_words():
     return words
 
 
-def banded_ops(n, band, seed, gadget_count=0, gadget_arity=3):
-    """Clifford body and tail of a seeded banded instance; returns ``(N, body, tail, ancillas)``."""
+def banded_ops(n, band, seed, gadget_count=0, gadget_arity=3, fp32_angles=False):
+    """Clifford body and tail of a seeded banded instance; returns ``(N, body, tail, ancillas)``.  With
+    ``fp32_angles`` the rotation angles are rounded to single precision (the same draws, each an exact float32
+    value), for workloads whose angles are float32 parameters."""
     rng = np.random.default_rng(seed)
     words = clifford_words()
     body = [("h", [q]) for q in range(n)]
@@ -70,6 +72,9 @@ def banded_ops(n, band, seed, gadget_count=0, gadget_arity=3):
         body += [(g, [q]) for g in words[int(cliff[q])]]
     alpha = rng.uniform(0, 2 * np.pi, size=n)
     beta = rng.uniform(0, 2 * np.pi, size=n)
+    if fp32_angles:
+        alpha = alpha.astype(np.float32).astype(np.float64)
+        beta = beta.astype(np.float32).astype(np.float64)
     anc = []
     for a in range(gadget_count):
         qs = sorted(int(q) for q in rng.choice(min(n, band), size=gadget_arity, replace=False))
@@ -98,9 +103,10 @@ class Instance:
     ``order`` defaults to the natural order (ancillas, if any, first: projected vertices must precede the sampled
     ones).  ``R`` defaults to the plan's width; a larger R sizes the arena as a workload asks."""
 
-    def __init__(self, n, band, seed, R=None, gadgets_count=0, order=None, keep_order="physical"):
+    def __init__(self, n, band, seed, R=None, gadgets_count=0, order=None, keep_order="physical", fp32_angles=False):
         self.n, self.band, self.seed = n, band, seed
-        self.N, self.body, self.tail, self.ancillas = banded_ops(n, band, seed, gadgets_count)
+        self.fp32_angles = fp32_angles
+        self.N, self.body, self.tail, self.ancillas = banded_ops(n, band, seed, gadgets_count, fp32_angles=fp32_angles)
         self.order = list(order) if order is not None else (list(range(n, self.N)) + list(range(n)))
         self.am = Amplitudes(self.N, tableau_of(self.N, self.body), self.tail, self.order)
         self.width = width(self.am.steps)
@@ -129,7 +135,7 @@ CONFIGS = {
     "c1": dict(n=24, band=10, seed=1, R=10),
     "c2": dict(n=70, band=30, seed=2, R=30),
     "c3": dict(n=70, band=26, seed=3),
-    "c4": dict(n=60, band=24, seed=4),
+    "c4": dict(n=60, band=24, seed=4, fp32_angles=True),
     "c5": dict(n=80, band=34, seed=5, R=34, gadgets_count=40),
 }
 

@@ -212,3 +212,38 @@ def test_c5_width34_sliced_twist_vs_unsliced(c5):
         _, lpf, _, _ = e.resume(b, K, tabs, lss, 5, 0, cs, ts)
     print(f"  sliced {lpf:.12f} unsliced {want:.12f}", flush=True)
     assert abs(lpf - want) <= 1e-10
+
+
+def test_tw70_twist_gate(golden, tmp_path):
+    """The reference's twist gate at 70 qubits (cloud_nq80/run_hybrid.sh:32-43): IBM #151 on its width-30 plan, the 3
+    proposals of cloud_nq80/tw70_props.txt (20 sampled trailing bits) drawn by unsliced hybrid (seed 17) and by the
+    two slice plans of vertex 21 (prefix to K = 50 on each, resume with the twist file): identical bits, log2 Pr
+    within 1e-5 (the gate's bound) in complex64 and within 1e-10 in complex128.  The slice plans and twist are the
+    ones this package writes, byte-identical to the reference's committed files (tests/test_sliced.py)."""
+    d = os.path.join(golden, "sliced")
+    ibm = os.path.join(golden, "ibm")
+    meta = json.load(open(os.path.join(d, "reference_sliced_plans.json")))["tw70"]
+    n, T, tail = egs.tableau.clifford_body(os.path.join(ibm, "nq70.qasm"))
+    order = json.load(open(os.path.join(ibm, "order_w30.json")))
+    am = egs.amp.Amplitudes(n, T, tail, order)
+    info = egs.sliced.make_sliced_plans(am.A, am.M, order, meta["slices"], str(tmp_path / "tw70"), K=meta["K"])
+    K, seed = meta["K"], meta["hybrid_seed"]
+    props = egs.planio.read_strings(os.path.join(d, "tw70_props.txt"), 70, allow_q=True)
+    cs = [c for c, _ in info["twists"]]
+    ts = [t for _, t in info["twists"]]
+    for storage, tol in ((gs.C64, 1e-5), (gs.C128, 1e-10)):
+        print(f"tw70 twist gate storage={'c128' if storage == gs.C128 else 'c64'}: unsliced hybrid", flush=True)
+        with gs.Engine(os.path.join(ibm, "nq70_inplace_plan.txt"), storage) as e:
+            hb, hlp, _ = e.hybrid(props, seed)
+        tabs = [[None] * len(info["plans"]) for _ in props]
+        lss = [[None] * len(info["plans"]) for _ in props]
+        for j, p in enumerate(info["plans"]):
+            with gs.Engine(p, storage) as e:
+                for i, row in enumerate(props):
+                    tabs[i][j], _, _, lss[i][j], _ = e.prefix(row, K)
+        print("  twisted two-slice resume", flush=True)
+        with gs.Engine(info["plans"][0], storage) as e:
+            for i, row in enumerate(props):
+                rb, lpf, _, _ = e.resume(row, K, tabs[i], lss[i], seed, i, cs, ts)
+                assert np.array_equal(rb, hb[i]), i
+                assert abs(lpf - hlp[i]) <= tol, (i, lpf, hlp[i])
