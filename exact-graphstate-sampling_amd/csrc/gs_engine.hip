@@ -18,6 +18,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -750,12 +751,19 @@ void free_engine(gs_engine* e) {
     delete e;
 }
 
+// Device time of the enqueued work between begin and end (engines without events -- the scratch-backed prob entry
+// point -- only drain the stream).
 int begin_timing(gs_engine* e) {
+    if (!e->ev0) return 0;
     HIP_TRY(hipEventRecord(e->ev0, e->stream));
     return 0;
 }
 
 int end_timing(gs_engine* e) {
+    if (!e->ev1) {
+        HIP_TRY(hipStreamSynchronize(e->stream));
+        return 0;
+    }
     HIP_TRY(hipEventRecord(e->ev1, e->stream));
     HIP_TRY(hipEventSynchronize(e->ev1));
     float ms = 0;
@@ -1376,3 +1384,105 @@ int gs_set_fusion(gs_engine* eng, int on) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------------------------------
+// prob with no device allocation (gs_prob_noalloc).  Every device buffer gs_prob's engine allocates -- the arena,
+// the step descriptors, the chain state, the sweeps' block partials and counter, the per-string staging -- is
+// carved out of `scratch`, one file-scope device array per device that lives until the process ends.  Host code
+// never uses the array's name as an address: the device address comes from hipGetSymbolAddress, looked up once per
+// device with that device current.  Calls that overlap on one device share the array and race.
+
+namespace {
+constexpr size_t kScratchBytes = size_t(64) << 20;     // 64 MiB per device
+constexpr int kMaxScratchDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxScratchDevices];              // device address of the array on each device (once found)
+
+// Device address of the scratch array on `device` (which must be current).
+int scratch_address(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void* p = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || !p) return fail(17, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+        g_scratch_addr[device] = p;
+    }
+    *out = static_cast<unsigned char*>(g_scratch_addr[device]);
+    return 0;
+}
+
+__global__ void clear_words_kernel(unsigned* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+}  // namespace
+
+extern "C" int gs_prob_noalloc(const gs_plan* plan, int storage, int device, const int8_t* bits, int64_t nstr,
+                               double* log2p) {
+    if (!plan || nstr < 0 || (nstr > 0 && (!bits || !log2p))) return fail(16, "bad argument");
+    if (storage != GS_C64 && storage != GS_C128) return fail(16, "storage must be GS_C64 or GS_C128");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(17, "no HIP device");
+    if (device < 0 || device >= ndev || device >= kMaxScratchDevices) return fail(16, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    gs_engine e;                                       // host-side plan of the launches; no device memory of its own
+    e.plan = plan->p;
+    e.storage = storage;
+    e.device = device;
+    e.esz = storage == GS_C64 ? 8 : 16;
+    const int ns = e.plan.ns;
+    if (e.plan.R > 40) return fail(16, "arena of 2^" + std::to_string(e.plan.R) + " entries exceeds the scratch buffer");
+    // carve the buffer (256-byte aligned pieces)
+    const size_t C = gs_engine::kChunk, nsz = size_t(std::max(ns, 1));
+    size_t used = 0;
+    auto take = [&](size_t n) {
+        const size_t at = used;
+        used += (std::max<size_t>(n, 16) + 255) & ~size_t(255);
+        return at;
+    };
+    const size_t o_ctl = take(sizeof(Ctl)), o_counter = take(sizeof(unsigned) * kCounters),
+                 o_part = take(sizeof(double) * kPartDoubles), o_out = take(sizeof(double) * C * 4),
+                 o_amp = take(16 * C), o_forced = take(C * nsz), o_U = take(sizeof(double) * C * nsz),
+                 o_bits = take(C * nsz), o_cond = take(sizeof(double) * C * nsz), o_plain = take(sizeof(KStep) * nsz),
+                 o_fused = take(sizeof(KStep) * nsz), o_arena = take((size_t(1) << e.plan.R) * e.esz);
+    if (used > kScratchBytes)
+        return fail(16, "the plan needs " + std::to_string(used) + " bytes of scratch; gs_prob_noalloc holds " +
+                            std::to_string(kScratchBytes) + " (arena 2^R entries: R <= 22 in complex64, 21 in complex128)");
+    unsigned char* base = nullptr;
+    if (int rc = scratch_address(device, &base)) return rc;
+    if (int rc = build_steps(&e)) return rc;
+    e.ctl = reinterpret_cast<Ctl*>(base + o_ctl);
+    e.counter = reinterpret_cast<unsigned*>(base + o_counter);
+    e.part = reinterpret_cast<double*>(base + o_part);
+    e.d_out = reinterpret_cast<double*>(base + o_out);
+    e.d_amp = base + o_amp;
+    e.d_forced = reinterpret_cast<int8_t*>(base + o_forced);
+    e.d_U = reinterpret_cast<double*>(base + o_U);
+    e.d_bits = reinterpret_cast<int8_t*>(base + o_bits);
+    e.d_cond = reinterpret_cast<double*>(base + o_cond);
+    e.dplain = reinterpret_cast<KStep*>(base + o_plain);
+    e.dfused = reinterpret_cast<KStep*>(base + o_fused);
+    e.arena = base + o_arena;
+    // host staging in ordinary (pageable) memory; the null stream; no timing events
+    std::vector<int8_t> h_forced(C * nsz);
+    std::vector<double> h_out(C * 4), h_cond(C * nsz);
+    std::vector<unsigned char> h_amp(16 * C);
+    e.h_forced = h_forced.data();
+    e.h_out = h_out.data();
+    e.h_cond = h_cond.data();
+    e.h_amp = h_amp.data();
+    if (ns > 0) {
+        HIP_TRY(hipMemcpy(e.dplain, e.hplain.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e.dfused, e.hfused.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice));
+    }
+    // the sweeps' block counter must start at 0 (a kernel fill, not a memset node: callers may capture this)
+    hipLaunchKernelGGL(clear_words_kernel, dim3(1), dim3(64), 0, e.stream, e.counter, kCounters);
+    HIP_TRY(hipGetLastError());
+    return run_forced(&e, kChain, bits, nstr, log2p, nullptr, nullptr);
+}

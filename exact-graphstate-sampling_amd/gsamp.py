@@ -25,7 +25,8 @@ ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy
                "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_set_xcd_items", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms", "gs_tt_plan_load", "gs_tt_plan_info",
                "gs_tt_plan_destroy", "gs_tt_engine_create", "gs_tt_engine_destroy", "gs_tt_prob", "gs_tt_chain",
-               "gs_tt_sample", "gs_tt_hybrid", "gs_tt_prefix", "gs_tt_resume", "gs_tt_dp", "gs_tt_last_error")
+               "gs_tt_sample", "gs_tt_hybrid", "gs_tt_prefix", "gs_tt_resume", "gs_tt_dp", "gs_tt_last_error",
+               "gs_prob_noalloc")
 
 _lib = None
 
@@ -59,6 +60,7 @@ def library():
         "gs_engine_create": (I, [P, I, I, pp]),
         "gs_engine_destroy": (None, [P]),
         "gs_prob": (I, [P, i8p, I64, dp]),
+        "gs_prob_noalloc": (I, [P, I, I, i8p, I64, dp]),
         "gs_chain": (I, [P, i8p, I64, dp, dp, dp]),
         "gs_sample": (I, [P, U64, I64, I64, I, i8p, dp]),
         "gs_hybrid": (I, [P, i8p, I64, U64, I64, i8p, dp, dp]),
@@ -158,6 +160,18 @@ class Plan:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.gs_plan_destroy(self._h)
             self._h = None
+
+
+def prob_noalloc(plan, bits, storage=C64, device=0):
+    """log2 Pr of forced strings like ``Engine.prob``, with no device allocation: the library's fixed 64 MiB device
+    buffer holds the arena (R <= 22 in complex64, 21 in complex128).  Concurrent calls on one device race."""
+    p = plan if isinstance(plan, Plan) else Plan(path=plan)
+    b = np.ascontiguousarray(np.atleast_2d(np.asarray(bits, dtype=np.int8)))
+    if b.shape[1] != p.ns:
+        raise ValueError(f"strings must have {p.ns} steps, got {b.shape[1]}")
+    out = np.empty(b.shape[0])
+    _check(library().gs_prob_noalloc(p._h, storage, device, _i8(b), b.shape[0], _f8(out)))
+    return out
 
 
 class Engine:

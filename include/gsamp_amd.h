@@ -147,6 +147,15 @@ int gs_tt_resume(gs_tt_engine* eng, const int8_t* bits, int K, int nslices, cons
 int gs_tt_dp(gs_tt_engine* eng, int64_t njobs, const int* nslices, const double* pre, const double* w, double* out);
 const char* gs_tt_last_error(void);
 
+/* prob mode without any device allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same inputs, outputs, results and kernels as gs_prob on an engine of the same plan and storage, but the arena,
+ * step descriptors, chain state and staging buffers live in one file-scope device array of 64 MiB on each device,
+ * held until the process ends; plans that need more (an arena of 2^R entries with R > 22 in complex64, R > 21 in
+ * complex128) are rejected with code 16.  Runs on the null stream of `device` and returns when it is done.  Every
+ * call on a device shares that device's array: calls that overlap on one device race. */
+int gs_prob_noalloc(const gs_plan* plan, int storage, int device, const int8_t* bits, int64_t nstr, double* log2p);
+
 /* Timing of the device work of the last call, in milliseconds (HIP events around the step kernels). */
 double gs_last_device_ms(const gs_engine* eng);
 
