@@ -78,7 +78,7 @@ def test_adjoint_finite_differences_and_segments(tmp_path):
         ga, gb = model.grad_ln_pr(bits, Z, G, dA, dB)
         loss_grad = -gb.mean(0)
         def loss_at(q, d):
-            beta = model.beta0.copy()
+            beta = model.beta0.astype(np.float64)          # the step in FP64 (the angles themselves are float32)
             beta[q] += d
             bt.set_matrices(0, model.matrices(model.alpha0, beta)[0])
             l, _, _ = bt.score([bits], want_cond=False, want_amp=False)
