@@ -29,7 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
-SWEEP_NAMES = ("gram", "direct", "write", "write+gram", "write+direct", "write+gram of a one-slot next step")
+SWEEP_NAMES = ("gram", "direct", "write", "write+gram", "write+direct", "write+gram of a one-slot next step",
+               "pattern sums of a run of diagonal steps", "product pass of a run of diagonal steps")
 ROUTES = ("m1", "split", "mixA", "mixB")
 
 
@@ -38,6 +39,10 @@ def sweep_bytes(step, sweep, esz):
     that fold in the next step's norms, which read nothing more) read the live slots and write the outputs."""
     ntq = len({t[0] for t in step["terms"]})
     read = (1 << step["nk"]) * ntq * esz
+    if sweep == 5:                      # pattern sums of a run: one read of its table
+        return read
+    if sweep == 6:                      # product pass of a run: read and write
+        return 2 * read
     if sweep >= 2:
         return read + (1 << step["nk"]) * (2 if step["m"] == 2 else 1) * esz
     return read
@@ -129,10 +134,12 @@ def main():
     agg = {}
     for k, s in enumerate(steps):
         gk = geo[k]
-        for sw in range(5):
+        for sw in range(7):
             if nl[k, sw] == 0:
                 continue
-            if sw >= 3 and gk["fvg"] == 2:
+            if sw >= 5:                                                           # a run of diagonal steps
+                key = (2, 1, 1, 0, sw + 1, 0)
+            elif sw >= 3 and gk["fvg"] == 2:
                 key = (1 << gk["flr"], 2, gk["ntq"], gk["route"], sw, gk["route2"])   # fused layout: 2 runs
             elif sw >= 3:
                 key = (1 << gk["flr"], 1, gk["ntq"], gk["route"], 5, 0)           # plain layout folding a 1-slot Gram
@@ -147,7 +154,8 @@ def main():
     key, dom = max(agg.items(), key=lambda kv: kv[1]["ms"])
     achieved = dom["bytes"] / (dom["ms"] / 1e3) / 1e9
     vr, vg, ntq, route, sw, route2 = key
-    kname = f"gs::sweep_kernel<float2, {vr}, {vg}, {ntq}, {route}, {sw}, {route2}>"
+    kname = (f"gs::sweep_kernel<float2, {vr}, {vg}, {ntq}, {route}, {sw}, {route2}>" if sw < 6 else
+             "gs::run_hist_kernel<float2>" if sw == 6 else "gs::run_apply_kernel<float2>")
     kdesc = (f"{vr * vg} cosets/thread ({vg} run(s) of {vr}), {ntq} slots/coset, route {ROUTES[route]}, "
              f"{SWEEP_NAMES[sw]} sweep" + (f", next route {ROUTES[route2]}" if sw in (3, 4) else ""))
     traffic_meas = profiled_traffic(kname)

@@ -76,6 +76,26 @@ struct TStep {
     int lg_iter;           // log2 tiles per block
 };
 
+// A run of consecutive diagonal steps: one-slot, one-output steps with slot offset 0 over the same live table.  Step i
+// of the run multiplies entry a by w0_i + (-1)^{s_i(a)} w1_i, s_i an affine parity of the address, so the run depends
+// on an entry only through its pattern sigma = (s_0 .. s_{j-1}): one pass sums |v|^2 per pattern, one block makes all
+// j decisions from those sums, one pass multiplies every entry by its pattern's product factor.  Coordinates: 0 the
+// unit physical bit (one aligned run of 2 entries per thread), 1..8 the unit bits 2..256 (the threads), then the
+// block coordinates, then the tile coordinates, chosen so that no tile coordinate changes sigma.  Every column but the
+// first has bit 0 clear.
+constexpr int kMaxRun = 8;              // steps per run: 2^8 sign patterns
+constexpr int kRunBins = 1 << kMaxRun;
+constexpr int kRunChunks = 64;          // first-stage reduction of the per-block pattern sums
+struct RStep {
+    uint64_t cols[40];                  // physical column of each coordinate
+    uint8_t sig[40];                    // change of sigma (bit i: step i) when the coordinate is set
+    double mre[kMaxRun][2][2];          // M[b][x] of each step of the run
+    double mim[kMaxRun][2][2];
+    int nk, j, k0;                      // coordinates, steps, first step
+    int kappa;                          // sigma of the coordinate origin (bit i: par of step i)
+    int lg_grid, lg_iter, xcd;
+};
+
 // Chain state carried on the device between steps (one per engine).
 struct Ctl {
     double prev;           // norm of the stored table (chain1 / run_steps bookkeeping)
@@ -93,8 +113,9 @@ struct Ctl {
 // Modes of the chain (how norms become a branch and bookkeeping).
 enum ChainMode { kChain = 0, kChain1 = 1, kRun = 2 };
 
-// What the last block of a sweep does after the sweep.
-enum EpiRole { kRoleNorm = 0, kRoleWrite = 1 };
+// What the last block of a sweep does after the sweep (kRoleStored: record the stored table's norm only -- the
+// product pass of a run of diagonal steps, whose decisions and bookkeeping were made before it).
+enum EpiRole { kRoleNorm = 0, kRoleWrite = 1, kRoleStored = 2 };
 // Write role: what follows step k.
 enum Next { kNextPublish = -2, kNextDecide = -1, kNextFusedGram = kGram, kNextFusedDirect = kDirect, kNextSweep = 3 };
 

@@ -36,6 +36,11 @@ void* publish_kernel_ptr();
 void* reduce_kernel_ptr(int sweep);
 void* small_run_kernel_ptr(int storage);
 int small_run_threads();
+void* reduce1_kernel_ptr();
+void* run_hist_kernel_ptr(int storage);
+void* run_apply_kernel_ptr(int storage);
+void* run_chunk_kernel_ptr();
+void* run_decide_kernel_ptr();
 }  // namespace gs
 
 using namespace gs;
@@ -256,6 +261,16 @@ struct gs_engine {
     int xcd_blocks = 1;             // XCD-contiguous block coordinates in every sweep
     int pair_ntq = 0;               // plain sweeps of steps with at most this many slots per coset take two runs per thread
     int pair_far = 0;               // ... the second one at the top coordinate (1) or the next unit bit (0)
+    // runs of diagonal steps (RStep): descriptors, the run starting at each step (-1: none), the pattern sums
+    std::vector<RStep> hrun;
+    std::vector<int> run_at;
+    RStep* drun = nullptr;
+    size_t drun_cap = 0;            // descriptors the device array holds
+    double* hpart = nullptr;        // [blocks][kRunBins] pattern sums of pass 1
+    double* hchunk = nullptr;       // [kRunChunks][kRunBins]
+    double2* rfac = nullptr;        // [kRunBins] product factors for pass 2
+    int run_max = kMaxRun;          // steps per run (< 2: runs off)
+    int run_min_nk = 20;            // runs only of steps with at least 2^run_min_nk cosets
     // optional per-sweep timing (bench / profiling)
     bool profile = false;
     std::vector<hipEvent_t> pev;
@@ -266,7 +281,7 @@ struct gs_engine {
 
 namespace {
 
-constexpr int kNumSweeps = 5;
+constexpr int kNumSweeps = 7;        // gram, direct, write, write+gram, write+direct, run pattern sums, run product
 
 int ns_of(const gs_engine* e) { return e->plan.ns; }
 
@@ -346,6 +361,80 @@ int launch_init(gs_engine* e) {
     return 0;
 }
 
+// The run of diagonal steps that starts at step k and ends by k1, or -1.
+int run_from(const gs_engine* e, int k, int k1) {
+    if (k >= int(e->run_at.size())) return -1;
+    const int ri = e->run_at[size_t(k)];
+    return ri >= 0 && k + e->hrun[size_t(ri)].j <= k1 ? ri : -1;
+}
+
+// A run of diagonal steps: pattern sums (one pass), their reduction, the run's decisions and bookkeeping (one block),
+// the product pass; its stored norm goes to the chain state.
+int launch_run(gs_engine* e, int ri, const Epi& base) {
+    const RStep& rs = e->hrun[size_t(ri)];
+    const RStep* rp = e->drun + ri;
+    void* mem = e->arena;
+    Ctl* ctl = e->ctl;
+    double* hp = e->hpart;
+    const double* hpc = e->hpart;
+    double* hc = e->hchunk;
+    const double* hcc = e->hchunk;
+    double2* fac = e->rfac;
+    const double2* facc = e->rfac;
+    double* part = e->part;
+    unsigned nblk = 1u << rs.lg_grid;
+    unsigned G = std::min<unsigned>(nblk, kRunChunks);
+    auto timed = [&](int tag, auto&& launch) -> int {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (e->profile) {
+            HIP_TRY(hipEventCreate(&a));
+            HIP_TRY(hipEventCreate(&b));
+            HIP_TRY(hipEventRecord(a, e->stream));
+        }
+        if (int rc = launch()) return rc;
+        if (e->profile) {
+            HIP_TRY(hipEventRecord(b, e->stream));
+            e->pev.push_back(a);
+            e->pev.push_back(b);
+            e->pev_tag.push_back(rs.k0 * 8 + tag);
+        }
+        return 0;
+    };
+    if (int rc = timed(5, [&]() -> int {
+            void* args[] = {&rp, &mem, &hp, &ctl};
+            HIP_TRY(hipLaunchKernel(run_hist_kernel_ptr(e->storage), dim3(nblk), dim3(kThreads), args, 0, e->stream));
+            return 0;
+        }))
+        return rc;
+    {
+        void* args[] = {&hpc, &nblk, &hc, &ctl};
+        HIP_TRY(hipLaunchKernel(run_chunk_kernel_ptr(), dim3(G), dim3(kRunBins), args, 0, e->stream));
+    }
+    Epi ed = base;
+    ed.k = rs.k0;
+    ed.role = kRoleWrite;
+    {
+        void* args[] = {&rp, &hcc, &G, &ctl, &ed, &fac};
+        HIP_TRY(hipLaunchKernel(run_decide_kernel_ptr(), dim3(1), dim3(kRunBins), args, 0, e->stream));
+    }
+    Epi ea = base;
+    ea.k = rs.k0 + rs.j - 1;
+    ea.role = kRoleStored;
+    if (int rc = timed(6, [&]() -> int {
+            void* args[] = {&rp, &mem, &facc, &part, &ctl, &ea};
+            HIP_TRY(hipLaunchKernel(run_apply_kernel_ptr(e->storage), dim3(nblk), dim3(kThreads), args, 0, e->stream));
+            return 0;
+        }))
+        return rc;
+    if (nblk > unsigned(kFenceBlocks)) {
+        const double* cpart = part;
+        int sw = kWrite;
+        void* rargs[] = {&cpart, &nblk, &ctl, &ea, &sw};
+        HIP_TRY(hipLaunchKernel(reduce1_kernel_ptr(), dim3(1), dim3(kThreads), rargs, 0, e->stream));
+    }
+    return 0;
+}
+
 // Which norms step k needs in a mode (-1: none -- forced, chain1, or a growth step).
 int norm_for(const gs_engine* e, int mode, int k, int forced_bit, bool forced_string) {
     const KStep& s = e->hplain[size_t(k)];
@@ -379,6 +468,12 @@ int enqueue_run(gs_engine* e, int mode, int k0, int k1, const int8_t* forced_h, 
     for (int k = k0; k < k1; k++) {
         int ke = k;                                       // a run of two or more small steps: one launch
         while (e->small_nk >= 0 && ke < k1 && e->hplain[size_t(ke)].nk <= e->small_nk) ke++;
+        if (const int ri = run_from(e, k, k1); ri >= 0) {   // a run of diagonal steps decides its own steps
+            if (int rc = launch_run(e, ri, base)) return rc;
+            decided = false;
+            k += e->hrun[size_t(ri)].j - 1;
+            continue;
+        }
         if (ke - k >= 2) {
             const KStep* sp = e->dplain;
             void* mem = e->arena;
@@ -412,7 +507,9 @@ int enqueue_run(gs_engine* e, int mode, int k0, int k1, const int8_t* forced_h, 
         } else {
             const int nn = norm_for(e, mode, k + 1, fb(k + 1), fstr);
             const auto& L = e->launch[size_t(k)];
-            if (nn < 0) {
+            if (run_from(e, k + 1, k1) >= 0) {
+                ep.next = kNextSweep;
+            } else if (nn < 0) {
                 ep.next = kNextDecide;
             } else if (e->allow_fusion && e->fusable[size_t(k)] && (nn == kGram ? L.wgram : L.wdirect)) {
                 ep.next = nn;
@@ -638,6 +735,172 @@ bool build_fused1(const gs_engine* e, int k, KStep& f) {
     return true;
 }
 
+// Basis of the null space of the rows (GF(2) masks over nk coordinates).
+std::vector<uint64_t> gf2_kernel(std::vector<uint64_t> rows, int nk) {
+    std::vector<int> piv;
+    size_t r = 0;
+    for (int col = 0; col < nk && r < rows.size(); col++) {
+        size_t p = r;
+        while (p < rows.size() && !((rows[p] >> col) & 1)) p++;
+        if (p == rows.size()) continue;
+        std::swap(rows[r], rows[p]);
+        for (size_t i = 0; i < rows.size(); i++)
+            if (i != r && ((rows[i] >> col) & 1)) rows[i] ^= rows[r];
+        piv.push_back(col);
+        r++;
+    }
+    std::vector<uint64_t> ker;
+    for (int f = 0; f < nk; f++) {
+        if (std::find(piv.begin(), piv.end(), f) != piv.end()) continue;
+        uint64_t v = uint64_t(1) << f;
+        for (size_t i = 0; i < r; i++)
+            if ((rows[i] >> f) & 1) v |= uint64_t(1) << piv[i];
+        ker.push_back(v);
+    }
+    return ker;
+}
+
+// Descriptor of the run of j diagonal steps from step k0 (see RStep), in the coordinates of step k0's table; false
+// when the unit bits 1 .. 256 are not all in the live table or too few sign-neutral tile coordinates exist.
+bool make_run(const gs_engine* e, int k0, int j, RStep& rs) {
+    const Plan& P = e->plan;
+    const StepInfo& s0 = P.steps[size_t(k0)];
+    const std::vector<uint64_t>& C0 = s0.cols;
+    const int nk = s0.nk;
+    // sign of step i at the address C0 c: parity(D_i & c) ^ par_i
+    std::vector<uint64_t> D(static_cast<size_t>(j));
+    int kappa = 0;
+    for (int i = 0; i < j; i++) {
+        const StepInfo& si = P.steps[size_t(k0 + i)];
+        uint64_t row = 0;
+        for (int q = 0; q < nk; q++) {
+            uint64_t cq = 0;
+            if (!gf2_solve(si.cols, C0[size_t(q)], cq)) return false;
+            row |= uint64_t(parity64(cq & si.rhoc)) << q;
+        }
+        D[size_t(i)] = row;
+        kappa |= (si.par[0] & 1) << i;
+    }
+    auto col_of = [&](uint64_t u) {
+        uint64_t a = 0;
+        for (int q = 0; q < nk; q++)
+            if ((u >> q) & 1) a ^= C0[size_t(q)];
+        return a;
+    };
+    Echelon ech;
+    std::vector<uint64_t> lowv;                       // coordinates 0..8: the unit physical bits 1 .. 256
+    for (int b = 0; b < 9; b++) {
+        uint64_t u = 0;
+        if (!gf2_solve(C0, uint64_t(1) << b, u) || !ech.add(u)) return false;
+        lowv.push_back(u);
+    }
+    const int tiles = nk - 9;
+    const int cap = nk >= kBigStepLog2 ? e->grid_cap1_big : e->grid_cap1;
+    const int lg_grid = std::min(tiles, cap), lg_iter = tiles - lg_grid;
+    // tile coordinates: in the null space of every D_i, with an even physical column
+    std::vector<uint64_t> ker = gf2_kernel(D, nk), tilev;
+    uint64_t odd = 0;
+    for (uint64_t v : ker) {
+        if (col_of(v) & 1) {
+            if (!odd) {
+                odd = v;
+                continue;
+            }
+            v ^= odd;
+        }
+        if (int(tilev.size()) < lg_iter && ech.add(v)) tilev.push_back(v);
+    }
+    if (int(tilev.size()) < lg_iter) return false;
+    std::vector<uint64_t> blockv;                     // block coordinates: the plan's where they raise the rank
+    for (int q = 0; q < nk && int(blockv.size()) < lg_grid; q++) {
+        uint64_t u = uint64_t(1) << q;
+        if (col_of(u) & 1) u ^= lowv[0];
+        if (ech.add(u)) blockv.push_back(u);
+    }
+    if (int(blockv.size()) != lg_grid) return false;
+    memset(&rs, 0, sizeof rs);
+    std::vector<uint64_t> order(lowv);
+    order.insert(order.end(), blockv.begin(), blockv.end());
+    order.insert(order.end(), tilev.begin(), tilev.end());
+    for (size_t c = 0; c < order.size(); c++) {
+        rs.cols[c] = col_of(order[c]);
+        int sg = 0;
+        for (int i = 0; i < j; i++) sg |= parity64(D[size_t(i)] & order[c]) << i;
+        rs.sig[c] = uint8_t(sg);
+    }
+    for (int i = 0; i < j; i++)
+        for (int b = 0; b < 2; b++)
+            for (int x = 0; x < 2; x++) {
+                rs.mre[i][b][x] = P.steps[size_t(k0 + i)].M[b][x].real();
+                rs.mim[i][b][x] = P.steps[size_t(k0 + i)].M[b][x].imag();
+            }
+    rs.nk = nk;
+    rs.j = j;
+    rs.k0 = k0;
+    rs.kappa = kappa;
+    rs.lg_grid = lg_grid;
+    rs.lg_iter = lg_iter;
+    rs.xcd = e->xcd_blocks;
+    // check the pattern bookkeeping against each step's own sign on pseudo-random coordinate vectors
+    uint64_t x = 0x9E3779B97F4A7C15ull ^ uint64_t(k0);
+    for (int t = 0; t < 64; t++) {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        uint64_t a = 0;
+        int sg = kappa;
+        for (int c = 0; c < nk; c++)
+            if ((x >> c) & 1) {
+                a ^= rs.cols[c];
+                sg ^= rs.sig[c];
+            }
+        for (int i = 0; i < j; i++) {
+            const StepInfo& si = P.steps[size_t(k0 + i)];
+            uint64_t ci = 0;
+            if (!gf2_solve(si.cols, a, ci) || (parity64(ci & si.rhoc) ^ si.par[0] ^ (sg >> i)) & 1) return false;
+        }
+    }
+    return true;
+}
+
+// Runs of at most run_max consecutive diagonal steps (one slot, one output, slot offset 0, the same live table) of
+// at least 2^run_min_nk cosets.
+void build_runs(gs_engine* e) {
+    const Plan& P = e->plan;
+    e->hrun.clear();
+    e->run_at.assign(size_t(std::max(P.ns, 1)), -1);
+    if (e->run_max < 2) return;
+    auto diag = [&](int k) {
+        const StepInfo& si = P.steps[size_t(k)];
+        return si.ntq == 1 && si.m == 1 && !si.growth && si.off[0] == 0 && si.tau0[0] == 0 && si.tau1[0] == 0 &&
+               si.nk >= std::max(10, e->run_min_nk);
+    };
+    auto same_table = [&](int a, int b) {
+        if (P.steps[size_t(a)].nk != P.steps[size_t(b)].nk) return false;
+        Echelon ech;
+        for (uint64_t c : P.steps[size_t(a)].cols) ech.add(c);
+        for (uint64_t c : P.steps[size_t(b)].cols)
+            if (ech.add(c)) return false;
+        return true;
+    };
+    for (int k = 0; k < P.ns;) {
+        if (!diag(k)) {
+            k++;
+            continue;
+        }
+        int end = k + 1;
+        while (end < P.ns && end - k < e->run_max && diag(end) && same_table(k, end)) end++;
+        RStep rs;
+        if (end - k >= 2 && make_run(e, k, end - k, rs)) {
+            e->run_at[size_t(k)] = int(e->hrun.size());
+            e->hrun.push_back(rs);
+            k = end;
+        } else {
+            k++;
+        }
+    }
+}
+
 int build_steps(gs_engine* e) {
     const Plan& P = e->plan;
     e->hplain.assign(size_t(P.ns), KStep{});
@@ -719,6 +982,35 @@ int build_steps(gs_engine* e) {
         e->hfused[size_t(k)] = f;
         e->fusable[size_t(k)] = 1;
     }
+    build_runs(e);
+    return 0;
+}
+
+// Copy the step and run descriptors to the device, growing the run arrays (and allocating the pattern-sum buffers the
+// first time the plan has a run).
+int upload_descriptors(gs_engine* e, bool logical) {
+    const size_t ns = size_t(e->plan.ns);
+    if (ns) {
+        HIP_TRY(hipMemcpy(e->dplain, e->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(e->dfused, e->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+        if (logical) HIP_TRY(hipMemcpy(e->dlogical, e->hlogical.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
+    }
+    if (e->hrun.empty()) return 0;
+    if (e->hrun.size() > e->drun_cap) {
+        (void)hipFree(e->drun);
+        e->drun = nullptr;
+        e->drun_cap = 0;
+        if (hipMalloc(&e->drun, sizeof(RStep) * e->hrun.size()) != hipSuccess)
+            return fail(11, "allocation of the run descriptors failed");
+        e->drun_cap = e->hrun.size();
+    }
+    if (!e->hpart) {
+        if (hipMalloc(&e->hpart, sizeof(double) * (size_t(kRunBins) << kMaxGridLog2)) != hipSuccess ||
+            hipMalloc(&e->hchunk, sizeof(double) * kRunChunks * kRunBins) != hipSuccess ||
+            hipMalloc(&e->rfac, sizeof(double2) * kRunBins) != hipSuccess)
+            return fail(11, "allocation of the run buffers failed");
+    }
+    HIP_TRY(hipMemcpy(e->drun, e->hrun.data(), sizeof(RStep) * e->hrun.size(), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -739,6 +1031,10 @@ void free_engine(gs_engine* e) {
     (void)hipFree(e->d_cond);
     (void)hipFree(e->d_out);
     (void)hipFree(e->d_amp);
+    (void)hipFree(e->drun);
+    (void)hipFree(e->hpart);
+    (void)hipFree(e->hchunk);
+    (void)hipFree(e->rfac);
     (void)hipHostFree(e->h_forced);
     (void)hipHostFree(e->h_U);
     (void)hipHostFree(e->h_bits);
@@ -922,10 +1218,7 @@ int gs_engine_create(const gs_plan* plan, int storage, int device, gs_engine** o
     if (hipStreamCreate(&e->stream) != hipSuccess || hipEventCreate(&e->ev0) != hipSuccess ||
         hipEventCreate(&e->ev1) != hipSuccess)
         return bail(fail(17, "stream/event creation failed"));
-    if (ns > 0 && (hipMemcpy(e->dplain, e->hplain.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess ||
-                   hipMemcpy(e->dfused, e->hfused.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess ||
-                   hipMemcpy(e->dlogical, e->hlogical.data(), sizeof(KStep) * size_t(ns), hipMemcpyHostToDevice) != hipSuccess))
-        return bail(fail(17, "plan upload failed"));
+    if (int rc = upload_descriptors(e, true)) return bail(rc);
     if (hipMemset(e->counter, 0, sizeof(unsigned) * kCounters) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
         return bail(fail(17, "counter initialisation failed"));
     e->prof_ms.assign(nsz * kNumSweeps, 0.0);
@@ -1278,13 +1571,7 @@ static int set_grid_caps(gs_engine* eng, int lg, bool one_slot_only) {
     eng->grid_cap1 = lg ? lg : kDefaultGridLog2One;
     eng->grid_cap1_big = lg ? lg : kDefaultGridLog2OneBig;
     if (int rc = build_steps(eng)) return rc;
-    const size_t ns = size_t(eng->plan.ns);
-    if (ns) {
-        HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(eng->dlogical, eng->hlogical.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
-    }
-    return 0;
+    return upload_descriptors(eng, true);
 }
 
 int gs_set_grid_log2(gs_engine* eng, int lg) { return set_grid_caps(eng, lg, false); }
@@ -1309,12 +1596,7 @@ int gs_set_fused_run(gs_engine* eng, int lr) {
     HIP_TRY(hipStreamSynchronize(eng->stream));
     eng->fused_run = lr;
     if (int rc = build_steps(eng)) return rc;
-    const size_t ns = size_t(eng->plan.ns);
-    if (ns) {
-        HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
-    }
-    return 0;
+    return upload_descriptors(eng, false);
 }
 
 // Plain sweeps of steps with at most ntq_max slots per coset (0: none, at most 2) take two runs per thread, the second
@@ -1328,12 +1610,7 @@ int gs_set_plain_pairs(gs_engine* eng, int ntq_max, int far) {
     eng->pair_ntq = ntq_max;
     eng->pair_far = far;
     if (int rc = build_steps(eng)) return rc;
-    const size_t ns = size_t(eng->plan.ns);
-    if (ns) {
-        HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
-    }
-    return 0;
+    return upload_descriptors(eng, false);
 }
 
 // Give each XCD a contiguous eighth of every sweep's block coordinates (on = 1) or deal them round-robin (0); results
@@ -1344,12 +1621,7 @@ int gs_set_xcd_blocks(gs_engine* eng, int on) {
     HIP_TRY(hipStreamSynchronize(eng->stream));
     eng->xcd_blocks = on ? 1 : 0;
     if (int rc = build_steps(eng)) return rc;
-    const size_t ns = size_t(eng->plan.ns);
-    if (ns) {
-        HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
-    }
-    return 0;
+    return upload_descriptors(eng, false);
 }
 
 // Tile order of the sweeps: 0 forward, 1 reverse on odd steps, 2 reverse on every step (results do not depend on it).
@@ -1360,12 +1632,7 @@ int gs_set_walk(gs_engine* eng, int mode) {
     HIP_TRY(hipStreamSynchronize(eng->stream));
     eng->walk = mode;
     if (int rc = build_steps(eng)) return rc;
-    const size_t ns = size_t(eng->plan.ns);
-    if (ns) {
-        HIP_TRY(hipMemcpy(eng->dplain, eng->hplain.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(eng->dfused, eng->hfused.data(), sizeof(KStep) * ns, hipMemcpyHostToDevice));
-    }
-    return 0;
+    return upload_descriptors(eng, false);
 }
 
 // Steps with at most 2^nk cosets run, in runs of two or more, in one single-block launch (nk -1: never; default 10).
@@ -1373,6 +1640,36 @@ int gs_set_small_steps(gs_engine* eng, int nk) {
     if (!eng) return fail(16, "null engine");
     if (nk < -1 || nk > 20) return fail(16, "small-step limit must be -1..20");
     eng->small_nk = nk;
+    return 0;
+}
+
+// Runs of at most max_steps consecutive diagonal steps (2..8; 0 or 1: off; default 8) of at least 2^min_nk cosets
+// (10..40; default 20) take one pattern-sum pass and one product pass instead of one pass per step.  Rebuilds the
+// descriptors (synchronous).
+int gs_set_diag_runs(gs_engine* eng, int max_steps, int min_nk) {
+    if (!eng) return fail(16, "null engine");
+    if (max_steps < 0 || max_steps > kMaxRun || min_nk < 10 || min_nk > kMaxKeep)
+        return fail(16, "max_steps must be 0.." + std::to_string(kMaxRun) + " and min_nk 10..40");
+    HIP_TRY(hipSetDevice(eng->device));
+    HIP_TRY(hipStreamSynchronize(eng->stream));
+    eng->run_max = max_steps;
+    eng->run_min_nk = min_nk;
+    if (int rc = build_steps(eng)) return rc;
+    return upload_descriptors(eng, false);
+}
+
+// The run of diagonal steps that step k belongs to: out[0] its first step, out[1] its length, out[2] log2 blocks of
+// its passes, out[3] log2 tiles per block (all -1 when step k is in no run).
+int gs_step_run(const gs_engine* eng, int k, int* out) {
+    if (!eng || k < 0 || k >= eng->plan.ns || !out) return fail(16, "bad step");
+    out[0] = out[1] = out[2] = out[3] = -1;
+    for (const RStep& r : eng->hrun)
+        if (k >= r.k0 && k < r.k0 + r.j) {
+            out[0] = r.k0;
+            out[1] = r.j;
+            out[2] = r.lg_grid;
+            out[3] = r.lg_iter;
+        }
     return 0;
 }
 
@@ -1432,6 +1729,7 @@ extern "C" int gs_prob_noalloc(const gs_plan* plan, int storage, int device, con
     if (device < 0 || device >= ndev || device >= kMaxScratchDevices) return fail(16, "device index out of range");
     HIP_TRY(hipSetDevice(device));
     gs_engine e;                                       // host-side plan of the launches; no device memory of its own
+    e.run_max = 0;                                     // no runs of diagonal steps (their pattern sums need 32 MB)
     e.plan = plan->p;
     e.storage = storage;
     e.device = device;

@@ -445,7 +445,10 @@ __device__ __forceinline__ void block_tree_sum(double (&sh)[NT][NQ], const doubl
 template <int NQ>
 __device__ void chain_step(const double (&red)[NQ], Ctl* ctl, const Epi& e, int sweep) {
     Ctl& c = *ctl;
-    if (e.role == kRoleNorm) {
+    if constexpr (NQ == 1) {            // one sum: the product pass of a run (role kRoleStored), the stored norm
+        c.nstored = red[0];
+        return;
+    } else if (e.role == kRoleNorm) {
         decide(c, e, e.k, sweep, red);
     } else {
         finish_write(c, e, e.k, red[0], red[1]);
@@ -873,6 +876,167 @@ __global__ __launch_bounds__(kSmallThreads) void small_run_kernel(const KStep* _
 }
 
 // ---------------------------------------------------------------------------------------------------------------
+// Runs of diagonal steps (RStep).  Pass 1 sums |v|^2 of the run's table per sign pattern sigma, pass 2 multiplies every
+// entry by the product factor of its pattern; between them one block reduces the sums and makes the run's j decisions
+// with the chain functions above, from the exact norms of each step's two branches:
+//   n_b(step i) = sum_sigma H[sigma] |F_{i-1}(sigma)|^2 |M_b0 + (-1)^{sigma_i} M_b1|^2,
+//   F_i(sigma) = F_{i-1}(sigma) (w0_i + (-1)^{sigma_i} w1_i),  (w0, w1) = the decided row of M_i times the step's scale,
+// which is what the per-step sweeps compute one step at a time (the values are rounded to the storage type once per
+// run instead of once per step).  All sums are in a fixed order: results are bitwise reproducible.
+
+// Thread and block parts of a run pass: base address and sigma of the thread's coordinate origin.
+__device__ __forceinline__ void run_origin(const RStep& r, uint64_t& a, int& sg, bool& active) {
+    const int tid = threadIdx.x;
+    a = 0;
+    sg = r.kappa;
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+        if ((tid >> b) & 1) {
+            a ^= r.cols[1 + b];
+            sg ^= r.sig[1 + b];
+        }
+    active = r.nk >= 9 || (uint64_t(tid) << 1) < (uint64_t(1) << r.nk);
+    const unsigned bid = r.xcd && r.lg_grid >= 3 ? (blockIdx.x >> 3) | ((blockIdx.x & 7u) << (r.lg_grid - 3))
+                                                 : blockIdx.x;
+    for (int b = 0; b < r.lg_grid; b++)
+        if ((bid >> b) & 1) {
+            a ^= r.cols[9 + b];
+            sg ^= r.sig[9 + b];
+        }
+}
+
+// Pass 1: per block, the sum of |v|^2 per pattern (2^j bins; threads own bins and scan the block's values in a fixed
+// order) into hpart[block][kRunBins].
+template <class S>
+__global__ __launch_bounds__(kThreads) void run_hist_kernel(const RStep* __restrict__ rp, const S* __restrict__ mem,
+                                                           double* __restrict__ hpart, const Ctl* __restrict__ ctl) {
+    if (ctl->dead) return;
+    const RStep& r = *rp;
+    uint64_t a_tb;
+    int sg;
+    bool active;
+    run_origin(r, a_tb, sg, active);
+    const int jit = 9 + r.lg_grid;
+    const int niter = 1 << r.lg_iter;
+    double acc0 = 0, acc1 = 0;
+    uint64_t a_it = 0;
+    for (int it = 0; it < niter; it++) {
+        if (it > 0) a_it ^= r.cols[jit + __builtin_ctz(unsigned(it))];
+        if (!active) continue;
+        S e[2];
+        Run<S, 2>::load(mem, a_tb ^ a_it, e);
+        add_norm(acc0, Cx<S>::widen(e[0]));
+        add_norm(acc1, Cx<S>::widen(e[1]));
+    }
+    __shared__ double sw[2 * kThreads];
+    __shared__ int ss[2 * kThreads];
+    const int t = threadIdx.x;
+    sw[2 * t] = acc0;
+    sw[2 * t + 1] = acc1;
+    ss[2 * t] = sg;
+    ss[2 * t + 1] = sg ^ r.sig[0];
+    __syncthreads();
+    double h = 0;
+    for (int q = 0; q < 2 * kThreads; q++)
+        if (ss[q] == t) h += sw[q];
+    hpart[size_t(blockIdx.x) * kRunBins + t] = h;
+}
+
+// First stage of the reduction: chunk g sums the pattern sums of blocks [g nblk / G, (g + 1) nblk / G) in order.
+__global__ __launch_bounds__(kRunBins) void run_chunk_kernel(const double* __restrict__ hpart, unsigned nblk,
+                                                            double* __restrict__ chunk, const Ctl* __restrict__ ctl) {
+    if (ctl->dead) return;
+    const unsigned G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
+    const unsigned b0 = unsigned((uint64_t(g) * nblk) / G), b1 = unsigned((uint64_t(g + 1) * nblk) / G);
+    double h = 0;
+    for (unsigned b = b0; b < b1; b++) h += hpart[size_t(b) * kRunBins + t];
+    chunk[size_t(g) * kRunBins + t] = h;
+}
+
+// The run's decisions and bookkeeping (one block of kRunBins threads, thread t = pattern t), then the product factors
+// F_j(sigma) for pass 2.  Each step: the branch norms by a fixed-order block sum, decide() (forced bit, or U < p1),
+// the written norm nb, finish_write(); the stored norm of pass 2 replaces nb as the growth-step norm afterwards.
+__global__ __launch_bounds__(kRunBins) void run_decide_kernel(const RStep* __restrict__ rp,
+                                                             const double* __restrict__ chunk, unsigned G,
+                                                             Ctl* __restrict__ ctl, Epi e, double2* __restrict__ fac) {
+    Ctl& c = *ctl;
+    if (c.dead) return;
+    const RStep& r = *rp;
+    const int t = threadIdx.x;
+    __shared__ double sh[kRunBins][2];
+    __shared__ int s_dead;
+    double H = 0;
+    for (unsigned g = 0; g < G; g++) H += chunk[size_t(g) * kRunBins + t];
+    double2 F = make_double2(1.0, 0.0);
+    double W = H;                                              // H |F|^2
+    if (t == 0) s_dead = 0;
+    for (int i = 0; i < r.j; i++) {
+        const int k = r.k0 + i;
+        const int s = (t >> i) & 1;
+        double nb2[2];
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const double2 m0 = make_double2(r.mre[i][b][0], r.mim[i][b][0]);
+            const double2 m1 = make_double2(r.mre[i][b][1], r.mim[i][b][1]);
+            nb2[b] = W * norm2(cadd(m0, negate_if(m1, s)));
+        }
+        block_tree_sum<2, kRunBins>(sh, nb2, t);
+        if (t == 0) {
+            const double red[2] = {sh[0][0], sh[0][1]};
+            decide(c, e, k, kDirect, red);
+            s_dead = c.dead;
+        }
+        __syncthreads();
+        if (s_dead) return;
+        const double2 w0 = make_double2(c.w[0], c.w[1]), w1 = make_double2(c.w[2], c.w[3]);
+        F = cmul(F, cadd(w0, negate_if(w1, s)));
+        W = H * norm2(F);
+        const double one[2] = {W, 0.0};
+        block_tree_sum<2, kRunBins>(sh, one, t);
+        if (t == 0) {
+            finish_write(c, e, k, sh[0][0], sh[0][0]);
+            if (!c.dead && k + 1 == e.k1) publish(c, e);
+            s_dead = c.dead;
+        }
+        __syncthreads();
+        if (s_dead) return;
+    }
+    fac[t] = F;
+}
+
+// Pass 2: every entry times its pattern's factor (FP64, one rounding), in place; the stored table's norm goes to the
+// chain state (role kRoleStored).
+template <class S>
+__global__ __launch_bounds__(kThreads) void run_apply_kernel(const RStep* __restrict__ rp, S* __restrict__ mem,
+                                                            const double2* __restrict__ fac, double* __restrict__ part,
+                                                            Ctl* __restrict__ ctl, Epi epi) {
+    if (ctl->dead) return;
+    const RStep& r = *rp;
+    uint64_t a_tb;
+    int sg;
+    bool active;
+    run_origin(r, a_tb, sg, active);
+    const double2 f0 = fac[sg], f1 = fac[sg ^ r.sig[0]];
+    const int jit = 9 + r.lg_grid;
+    const int niter = 1 << r.lg_iter;
+    double acc[1] = {0};
+    uint64_t a_it = 0;
+    for (int it = 0; it < niter; it++) {
+        if (it > 0) a_it ^= r.cols[jit + __builtin_ctz(unsigned(it))];
+        if (!active) continue;
+        const uint64_t a = a_tb ^ a_it;
+        S e[2];
+        Run<S, 2>::load(mem, a, e);
+        e[0] = Cx<S>::narrow(cmul(Cx<S>::widen(e[0]), f0));
+        e[1] = Cx<S>::narrow(cmul(Cx<S>::widen(e[1]), f1));
+        Run<S, 2>::store(mem, a, e);
+        add_norm(acc[0], Cx<S>::widen(e[0]));
+        add_norm(acc[0], Cx<S>::widen(e[1]));
+    }
+    sweep_epilogue<1>(acc, part, ctl, epi, kWrite);
+}
+
+// ---------------------------------------------------------------------------------------------------------------
 // Two-table (gather-form) steps (gsamp.cpp:21-29, :39-55; dp.cpp:44-53).  Output cp gathers its 2^nker inputs; S0 sums
 // the x = 0 terms, S1 the x = 1 terms with sign (-1)^{parity(c & rho)}; the outputs are M[b][0] S0 + M[b][1] S1.
 // Norm pass (SWEEP = kDirect): both branch norms |M[b][0] S0 + M[b][1] S1|^2 summed directly (the reference's chain).
@@ -1067,6 +1231,17 @@ void* reduce_kernel_ptr(int sweep) {
     return nullptr;
 }
 void* publish_kernel_ptr() { return reinterpret_cast<void*>(&publish_kernel); }
+void* reduce1_kernel_ptr() { return reinterpret_cast<void*>(&sweep_reduce_kernel<1>); }
+void* run_hist_kernel_ptr(int storage) {
+    return storage == 0 ? reinterpret_cast<void*>(&run_hist_kernel<float2>)
+                        : reinterpret_cast<void*>(&run_hist_kernel<double2>);
+}
+void* run_apply_kernel_ptr(int storage) {
+    return storage == 0 ? reinterpret_cast<void*>(&run_apply_kernel<float2>)
+                        : reinterpret_cast<void*>(&run_apply_kernel<double2>);
+}
+void* run_chunk_kernel_ptr() { return reinterpret_cast<void*>(&run_chunk_kernel); }
+void* run_decide_kernel_ptr() { return reinterpret_cast<void*>(&run_decide_kernel); }
 int small_run_threads() { return kSmallThreads; }
 // Two-table kernels: sweep kDirect (norm pass) or kWrite; storage 0 = complex64, 1 = complex128.
 void* gather_kernel_ptr(int storage, int sweep) {

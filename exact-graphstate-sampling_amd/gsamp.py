@@ -21,7 +21,7 @@ SEED_PLAIN, SEED_SEQ = 0, 1
 ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy", "gs_engine_create",
                "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
                "gs_last_device_ms", "gs_last_error", "gs_profile_enable", "gs_profile_read",
-               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_grid_log2_one_slot", "gs_set_small_steps", "gs_set_walk", "gs_set_xcd_blocks", "gs_set_plain_pairs", "gs_set_fused_run", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
+               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_grid_log2_one_slot", "gs_set_small_steps", "gs_set_walk", "gs_set_xcd_blocks", "gs_set_plain_pairs", "gs_set_fused_run", "gs_set_diag_runs", "gs_step_run", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
                "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_set_xcd_items", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms", "gs_tt_plan_load", "gs_tt_plan_info",
                "gs_tt_plan_destroy", "gs_tt_engine_create", "gs_tt_engine_destroy", "gs_tt_prob", "gs_tt_chain",
@@ -81,6 +81,8 @@ def library():
         "gs_set_walk": (I, [P, I]),
         "gs_set_plain_pairs": (I, [P, I, I]),
         "gs_set_fused_run": (I, [P, I]),
+        "gs_set_diag_runs": (I, [P, I, I]),
+        "gs_step_run": (I, [P, I, ip]),
         "gs_copy_probe": (I, [I, I64, dp]),
         "gs_batch_create": (I, [ctypes.POINTER(P), ip, I, I, I, pp]),
         "gs_batch_set_matrices": (I, [P, I, dp]),
@@ -297,14 +299,16 @@ class Engine:
     def profile(self, on=True):
         _check(library().gs_profile_enable(self._h, 1 if on else 0))
 
-    SWEEPS = ("gram", "direct", "write", "write+gram", "write+direct")
+    SWEEPS = ("gram", "direct", "write", "write+gram", "write+direct", "run sums", "run product")
 
     def profile_read(self):
-        """``(ms [ns, 5], launches [ns, 5])`` per step and sweep kind (``SWEEPS``) since profiling was enabled."""
-        ms = np.zeros(self.ns * 5)
-        n = np.zeros(self.ns * 5, np.int64)
+        """``(ms [ns, 7], launches [ns, 7])`` per step and sweep kind (``SWEEPS``) since profiling was enabled (a run
+        of diagonal steps counts at its first step)."""
+        nk = len(self.SWEEPS)
+        ms = np.zeros(self.ns * nk)
+        n = np.zeros(self.ns * nk, np.int64)
         _check(library().gs_profile_read(self._h, _f8(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
-        return ms.reshape(self.ns, 5), n.reshape(self.ns, 5)
+        return ms.reshape(self.ns, nk), n.reshape(self.ns, nk)
 
     def geometry(self, k):
         """Launch geometry of step k: dict(lr, grid, ntq, route, fusable, fgrid, route2, lg_iter, flr, flg_iter, vg,
@@ -319,6 +323,17 @@ class Engine:
         out = np.zeros(40, np.uint64)
         nk = library().gs_step_columns(self._h, k, 1 if fused else 0, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 40)
         return None if nk < 0 else [int(x) for x in out[:nk]]
+
+    def set_diag_runs(self, max_steps, min_nk=20):
+        """Runs of at most max_steps (2..8; 0: off) consecutive diagonal steps of at least 2^min_nk cosets take one
+        pattern-sum pass and one product pass (default 8, 20)."""
+        _check(library().gs_set_diag_runs(self._h, int(max_steps), int(min_nk)))
+
+    def run_of(self, k):
+        """``(first step, steps, log2 blocks, log2 tiles)`` of the run of diagonal steps holding step k, or None."""
+        out = np.zeros(4, np.int32)
+        _check(library().gs_step_run(self._h, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return None if out[0] < 0 else tuple(int(x) for x in out)
 
     def set_fusion(self, on):
         """Fold each sampled step's norm pass into the previous write sweep where possible (default on)."""

@@ -160,8 +160,9 @@ int gs_prob_noalloc(const gs_plan* plan, int storage, int device, const int8_t* 
 double gs_last_device_ms(const gs_engine* eng);
 
 /* ---- measurement and tuning (no counterpart in the reference; results do not depend on them). */
-/* Per-sweep timing: enable/disable, then read ms[ns][5] and launches[ns][5] accumulated since enabling
- * (sweeps: 0 Gram norms, 1 direct norms, 2 write, 3 write + next Gram, 4 write + next direct). */
+/* Per-sweep timing: enable/disable, then read ms[ns][7] and launches[ns][7] accumulated since enabling
+ * (sweeps: 0 Gram norms, 1 direct norms, 2 write, 3 write + next Gram, 4 write + next direct, 5 pattern sums and
+ * 6 product pass of a run of diagonal steps, counted at the run's first step). */
 int gs_profile_enable(gs_engine* eng, int on);
 int gs_profile_read(const gs_engine* eng, double* ms, int64_t* launches);
 /* Launch geometry of step k, out[12]: log2 cosets per run, blocks, slots per coset, route, fusable, blocks of the
@@ -190,6 +191,12 @@ int gs_set_xcd_blocks(gs_engine* eng, int on);
 /* Plain sweeps of steps with at most ntq_max slots per coset (0..2; default 0, off) take two runs per thread, the
    second at the top coordinate (far = 1) or at the next unit bit (far = 0). */
 int gs_set_plain_pairs(gs_engine* eng, int ntq_max, int far);
+/* Runs of at most max_steps (2..8; 0: off; default 8) consecutive diagonal steps -- one slot, one output, slot offset
+   0, the same live table -- of at least 2^min_nk cosets (10..40; default 20): one pass sums |v|^2 per pattern of the
+   run's signs, one block makes the run's decisions, one pass multiplies every entry by its pattern's factor. */
+int gs_set_diag_runs(gs_engine* eng, int max_steps, int min_nk);
+/* The run of diagonal steps holding step k, out[4]: first step, steps, log2 blocks, log2 tiles per block (-1: none). */
+int gs_step_run(const gs_engine* eng, int k, int* out);
 /* Runs of two or more steps with at most 2^nk cosets go to one single-block launch (nk -1: never; default 10). */
 int gs_set_small_steps(gs_engine* eng, int nk);
 /* One plain device copy of `bytes` (read + write), *ms its duration: calibrates hardware byte counters. */

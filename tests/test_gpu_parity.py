@@ -416,3 +416,55 @@ def test_one_slot_folded_norms_vs_oracle(gadget_plan, tmp_path, storage):
         assert close(cond, [[float(x) for x in r[2:]] for r in ref], 0) <= max(TOL[storage], 1e-9)
     assert np.array_equal(got[True][0], got[False][0])
     assert close(got[True][1], got[False][1], 0) <= TOL[storage]
+
+
+@pytest.mark.parametrize("storage", [gs.C64, gs.C128])
+@pytest.mark.parametrize("grid_log2", [0, 4])
+def test_diag_runs_vs_oracle(gadget_plan, tmp_path, storage, grid_log2):
+    """Steps 26..37 of the gadget instance are 12 consecutive diagonal steps (one slot, one output, slot offset 0,
+    2^18 cosets).  Taken as runs (8 + 4 steps: a pattern-sum pass, the decisions in one block, a product pass): hybrid
+    samples, prob and chain of random strings, and a prefix dump at K = 30 (inside a run, so those steps go one by one)
+    equal the oracle -- bits identical, complex128 within 1e-12, complex64 within 1e-5 -- with 2^9 blocks and one tile
+    each, and with 2^4 blocks looping over 32 sign-neutral tile coordinates."""
+    import subprocess
+    path, inst = gadget_plan
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    ns, na = inst.N, inst.N - inst.n
+    hy = np.full((4, ns), -1, np.int8)
+    hy[:, :na] = 0
+    rnd = np.random.default_rng(31).integers(0, 2, size=(4, ns)).astype(np.int8)
+    rnd[:, :na] = 0
+    egs.planio.write_strings(str(tmp_path / "hy.txt"), hy)
+    egs.planio.write_strings(str(tmp_path / "rnd.txt"), rnd)
+    cmd = [os.path.join(root, "oracle", "gs_oracle")] + (["--c128"] if storage == gs.C128 else []) + [path]
+    run = lambda *a: [l.split() for l in subprocess.run(cmd + [str(x) for x in a], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines() if l.strip()]
+    ref_h = run("hybrid", str(tmp_path / "hy.txt"), 13)
+    ref_p = [float(r[0]) for r in run("prob", str(tmp_path / "rnd.txt"))]
+    ref_c = run("chain", str(tmp_path / "rnd.txt"))
+    run("prefix", str(tmp_path / "rnd.txt"), 30, str(tmp_path / "d"))
+    tol = TOL[storage]
+    print(f"gadget n=48: runs of diagonal steps, storage={TAG[storage]} grid_log2={grid_log2}", flush=True)
+    with gs.Engine(path, storage) as e:
+        e.set_diag_runs(8, 10)
+        if grid_log2:
+            e.set_grid_log2(grid_log2)
+        runs = {e.run_of(k) for k in range(ns)} - {None}
+        assert {(r[0], r[1]) for r in runs} == {(26, 8), (34, 4)}, runs
+        assert all(r[3] == (5 if grid_log2 else 0) for r in runs)
+        ob, lph, condh = e.hybrid(hy, 13)
+        lpp = e.prob(rnd)
+        lpc, cond, _ = e.chain(rnd)
+        tab, nk, ntq, ls, _ = e.prefix(rnd[0], 30)
+        e.set_diag_runs(0)
+        ob0, lph0, _ = e.hybrid(hy, 13)
+    assert [egs.planio.format_bits(b) for b in ob] == [r[0] for r in ref_h]
+    assert close(lph, [float(r[1]) for r in ref_h], 0) <= tol
+    assert close(condh, [[float(x) for x in r[2:]] for r in ref_h], 0) <= max(tol, 1e-9)
+    assert close(lpp, ref_p, 0) <= tol
+    assert close(lpc, [float(r[0]) for r in ref_c], 0) <= tol
+    assert close(cond, [[float(x) for x in r[1:]] for r in ref_c], 0) <= max(tol, 1e-9)
+    dump, dnk, dntq, dls = egs.planio.read_dump(str(tmp_path / "d_0.bin"), c128=storage == gs.C128)
+    assert (dnk, dntq) == (nk, ntq) and abs(dls - ls) <= max(tol, 1e-12)
+    assert np.max(np.abs(np.asarray(dump) - tab)) <= 10 * tol * max(1.0, np.max(np.abs(tab)))
+    assert np.array_equal(ob0, ob) and close(lph0, lph, 0) <= tol

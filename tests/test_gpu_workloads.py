@@ -247,3 +247,22 @@ def test_tw70_twist_gate(golden, tmp_path):
                 rb, lpf, _, _ = e.resume(row, K, tabs[i], lss[i], seed, i, cs, ts)
                 assert np.array_equal(rb, hb[i]), i
                 assert abs(lpf - hlp[i]) <= tol, (i, lpf, hlp[i])
+
+
+def test_c5_width34_diag_runs_vs_steps(c5):
+    """The width-34 instance's 30 diagonal steps (runs of 8, 8, 6, 5 and 3 steps of 2^32-2^34 cosets) against the same
+    steps one sweep each, complex128 arena: prob of 2 strings within 1e-10; the runs are on by default."""
+    d, inst, path = c5
+    ns, na = inst.N, inst.N - inst.n
+    rnd = np.random.default_rng(57).integers(0, 2, size=(2, ns)).astype(np.int8)
+    rnd[:, :na] = 0
+    print("c5 R=34 c128: runs of diagonal steps against one sweep per step", flush=True)
+    with gs.Engine(path, gs.C128) as e:
+        runs = sorted({e.run_of(k) for k in range(ns)} - {None})
+        assert sum(r[1] for r in runs) >= 25, runs
+        lp_runs = e.prob(rnd)
+        e.set_diag_runs(0)
+        assert all(e.run_of(k) is None for k in range(ns))
+        lp_steps = e.prob(rnd)
+    print(f"  runs {runs}; max |d| {maxdiff(lp_runs, lp_steps):.2e}", flush=True)
+    assert maxdiff(lp_runs, lp_steps) <= 1e-10
