@@ -519,9 +519,20 @@ def leg_c5(gs, egs):
         t0 = time.perf_counter()
         e.chain(rnd)
         tc = (time.perf_counter() - t0) / 2
+        e.profile(True)                                         # one more sample with per-kernel events
+        e.hybrid(hy[:1], 5)
+        ms, nl = e.profile_read()
+        e.profile(False)
+    moved = sum(sweep_bytes(steps[k], sw, 8) * nl[k, sw] for k in range(ns) for sw in range(nl.shape[1]))
+    run_ms = float(ms[:, 5:].sum())
+    run_bytes = sum(sweep_bytes(steps[k], sw, 8) * nl[k, sw] for k in range(ns) for sw in (5, 6))
     res = {"workload": "c5 width 34: n=80 band 34 seed 5 + 40 gadget ancillas, R=34, complex64", "R": R,
-           "s_per_sample": ts, "hbm_gbps_algorithmic": traffic["sample"] / ts / 1e9, "s_per_string_chain": tc,
-           "chain_gbps_algorithmic": traffic["chain1"] / tc / 1e9, "samples": 2, "strings": 2}
+           "s_per_sample": ts, "hbm_gbps_algorithmic": traffic["sample"] / ts / 1e9,
+           "gb_per_sample_moved": moved / 1e9, "hbm_gbps_moved": moved / ts / 1e9,
+           "runs_of_diagonal_steps": {"ms_per_sample": run_ms, "gb_per_sample": run_bytes / 1e9,
+                                      "gbps": run_bytes / (run_ms / 1e3) / 1e9 if run_ms else None},
+           "s_per_string_chain": tc, "chain_gbps_algorithmic": traffic["chain1"] / tc / 1e9, "samples": 2,
+           "strings": 2}
     try:
         with gs.Engine(path, gs.C128) as e2:
             lp2 = e2.prob(ob)
@@ -609,18 +620,19 @@ def _cpu_model():
         return ""
 
 
-def _ref_sample_s(exe, plan, seed, reps, env):
+def _ref_sample_s(exe, plan, seed, reps, env, n2=2):
     """Seconds per exact sample of the reference executor, excluding plan parsing, arena allocation and first touch:
-    the wall time of ``sample 2`` minus that of ``sample 1`` (both from ``seed``), median of ``reps`` pairs."""
+    (wall time of ``sample n2`` - wall time of ``sample 1``) / (n2 - 1), both from ``seed``, median of ``reps``
+    pairs (n2 > 2 for plans whose samples take less than the process start-up)."""
     diffs = []
     for _ in range(reps):
         t = []
-        for n in (1, 2):
+        for n in (1, n2):
             t0 = time.perf_counter()
             subprocess.run([exe, plan, "sample", str(n), str(seed)], capture_output=True, text=True, env=env,
                            check=True, timeout=600)
             t.append(time.perf_counter() - t0)
-        diffs.append(t[1] - t[0])
+        diffs.append((t[1] - t[0]) / (n2 - 1))
     return float(np.median(diffs)), [float(x) for x in diffs]
 
 
@@ -645,14 +657,15 @@ def cpu_baseline(plan_path, seed, reps):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     env = dict(os.environ, OMP_NUM_THREADS=str(cores))
     c1 = os.path.join(ROOT, "tests", "golden", "c1", "plan.txt")
-    detail = {"cores": cores, "cpu": _cpu_model(), "method": "reference: wall(sample 2) - wall(sample 1), median of "
+    detail = {"cores": cores, "cpu": _cpu_model(), "method": "reference: wall(sample 2) - wall(sample 1) (n = 70 "
+              "plan) or (wall(sample 2049) - wall(sample 1)) / 2048 (n = 24 plan), median of "
               f"{reps} pairs; port: its per-sample compute time, median of 3 samples"}
     head = None
     try:
-        for tag, plan in (("c2", plan_path), ("c1", c1)):
+        for tag, plan, n2 in (("c2", plan_path, 2), ("c1", c1, 2049)):
             for storage, exe in (("c64", ref64), ("c128", ref128)):
                 if os.path.exists(exe):
-                    v, all_ = _ref_sample_s(exe, plan, seed, reps, env)
+                    v, all_ = _ref_sample_s(exe, plan, seed, reps, env, n2)
                     detail[f"reference_{tag}_{storage}_s"] = v
                     detail[f"reference_{tag}_{storage}_pairs_s"] = all_
                 if os.path.exists(port):
