@@ -130,6 +130,14 @@ struct Epi {
     double* out;               // this string's scalars: logp, logscale, prev, dead
     unsigned* counter;         // blocks finished (in-kernel reduction; the last block resets it)
     const KStep* plain;        // plain descriptors (M, growth flags) indexed by step
+    // The block partials a previous sweep left unreduced (pend_nblk 0: none).  Every block of this sweep reduces them
+    // in the same fixed order and runs their chain step (step pend_k, its role and next) on a private copy of the
+    // chain state read from the kernel's ctl; block 0 stores the result at ctl_out, the state the stream uses next.
+    const double* pend_part;
+    Ctl* ctl_out;
+    unsigned pend_nblk;
+    int pend_nq, pend_k, pend_role, pend_next, pend_sweep;
+    int quiet;                 // 1: leave the per-string outputs (bits, cond, out) to another block
 };
 
 }  // namespace gs

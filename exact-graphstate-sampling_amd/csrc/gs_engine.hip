@@ -157,6 +157,62 @@ __global__ __launch_bounds__(kThreads) void copy_probe_kernel(const float4* __re
         b[i] = a[i];
 }
 
+// Race check of one step's coset layout (the debug mode of gs_race_check), with the arena as an owner map of 64-bit
+// words (word index = slot * stride).  Coset c' (c' < 2^nk, in the layout's own coordinates) has base address
+// XOR_j c'_j cols[j]; it reads base ^ off[t] (t < ntq) and writes base, and base ^ g when m = 2.
+//   mode 0: clear every slot the layout reads or writes;
+//   mode 1: claim each read slot for its coset (owner c' + 1); a slot already claimed counts in cnt[0];
+//   mode 2: mark each written slot (bit 63); a slot marked twice counts in cnt[1], a slot claimed by another coset's
+//           read in cnt[2];
+//   mode 3: a run of diagonal steps (rp): mark each slot its passes visit (bit 62); a slot visited twice counts in
+//           cnt[1], a slot no coset of the step claimed (outside the step's table) in cnt[2].
+// cnt[3] counts the slots checked.
+constexpr uint64_t kOwnWrite = uint64_t(1) << 63, kOwnVisit = uint64_t(1) << 62;
+__global__ void race_kernel(const KStep* __restrict__ sp, const RStep* __restrict__ rp, unsigned long long* own,
+                            int stride, int mode, unsigned long long* cnt) {
+    const int nk = mode == 3 ? rp->nk : sp->nk;
+    const uint64_t* cols = mode == 3 ? rp->cols : sp->cols;
+    unsigned long long c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+    for (uint64_t c = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; c < (uint64_t(1) << nk);
+         c += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t base = 0;
+        for (int j = 0; j < nk; j++)
+            if ((c >> j) & 1) base ^= cols[j];
+        if (mode == 3) {
+            const unsigned long long old = atomicOr(own + base * stride, (unsigned long long)kOwnVisit);
+            c1 += (old & kOwnVisit) != 0;
+            c2 += (old & ~(kOwnVisit | kOwnWrite)) == 0;
+            c3++;
+            continue;
+        }
+        const KStep& s = *sp;
+        if (mode == 0) {
+            for (int t = 0; t < s.ntq; t++) own[(base ^ s.off[t]) * stride] = 0;
+            own[base * stride] = 0;
+            if (s.m == 2) own[(base ^ s.g) * stride] = 0;
+        } else if (mode == 1) {
+            for (int t = 0; t < s.ntq; t++) {
+                const unsigned long long old = atomicCAS(own + (base ^ s.off[t]) * stride, 0ull, c + 1);
+                c0 += old != 0;
+                c3++;
+            }
+        } else {
+            for (int t = 0; t < s.m; t++) {
+                const uint64_t a = t ? base ^ s.g : base;
+                const unsigned long long old = atomicOr(own + a * stride, (unsigned long long)kOwnWrite);
+                c1 += (old & kOwnWrite) != 0;
+                const unsigned long long who = old & ~(kOwnWrite | kOwnVisit);
+                c2 += who != 0 && who != c + 1;
+                c3++;
+            }
+        }
+    }
+    if (c0) atomicAdd(cnt + 0, c0);
+    if (c1) atomicAdd(cnt + 1, c1);
+    if (c2) atomicAdd(cnt + 2, c2);
+    if (c3) atomicAdd(cnt + 3, c3);
+}
+
 int route_and_relabel(const StepInfo& si, KStep& ks, std::string& err) {
     // canonical routing (see Route): reorder the offsets so that the term routing is one of four fixed patterns
     const int n = si.ntq;
@@ -232,9 +288,20 @@ struct gs_engine {
         int fgrid = 1;
     };
     std::vector<Launch> launch;
-    double* part = nullptr;         // per-block partials of the running sweep
+    double* part = nullptr;         // per-block partials of the running sweep (part_buf[0])
+    double* part_buf[2] = {nullptr, nullptr};   // alternate between sweeps: one may hold partials still unreduced
+    int cur_part = 0;               // the buffer the next sweep writes
     unsigned* counter = nullptr;    // blocks finished in the running sweep
-    Ctl* ctl = nullptr;
+    Ctl* ctl = nullptr;             // the current chain state (one of ctl_base[0..1])
+    Ctl* ctl_base = nullptr;        // two chain states: a sweep that reduces pending partials writes the other one
+    int cur_ctl = 0;
+    // block partials the last sweep left unreduced (pend_nblk 0: none): the next sweep reduces them itself, or a
+    // one-block kernel does before anything else runs
+    bool fold_reduce = false;       // measured slower on the width-26 plan: 4.560 against 4.484 ms per sample
+    const double* pend_part = nullptr;
+    unsigned pend_nblk = 0;
+    int pend_sweep = 0;
+    Epi pend_epi{};
     // per-string buffers, kChunk strings
     static constexpr int kChunk = 64;
     int8_t* d_forced = nullptr;
@@ -285,6 +352,23 @@ constexpr int kNumSweeps = 7;        // gram, direct, write, write+gram, write+d
 
 int ns_of(const gs_engine* e) { return e->plan.ns; }
 
+// Partial sums per block of each sweep kind (Gram 4, direct 2, write 2, write + Gram 6, write + direct 4).
+int partials_of(int sweep) { return sweep == kGram ? 4 : sweep == kWriteGram ? 6 : sweep == kWriteDirect ? 4 : 2; }
+
+// Reduce the partials the last sweep left (one-block kernel) before a launch that does not reduce them itself.
+int flush_pending(gs_engine* e) {
+    if (!e->pend_nblk) return 0;
+    const double* cpart = e->pend_part;
+    unsigned nblk = e->pend_nblk;
+    Ctl* ctl = e->ctl;
+    Epi ep = e->pend_epi;
+    int sw = e->pend_sweep;
+    e->pend_nblk = 0;
+    void* rargs[] = {&cpart, &nblk, &ctl, &ep, &sw};
+    HIP_TRY(hipLaunchKernel(reduce_kernel_ptr(sw), dim3(1), dim3(kThreads), rargs, 0, e->stream));
+    return 0;
+}
+
 int launch_sweep(gs_engine* e, int k, int sweep, const Epi& epi) {
     const auto& L = e->launch[size_t(k)];
     void* fn = sweep == kGram ? L.gram : sweep == kDirect ? L.direct : sweep == kWrite ? L.write
@@ -294,9 +378,24 @@ int launch_sweep(gs_engine* e, int k, int sweep, const Epi& epi) {
     const KStep* sp = (fused ? e->dfused : e->dplain) + k;
     const int grid = fused ? L.fgrid : L.grid;
     void* mem = e->arena;
-    double* part = e->part;
-    Ctl* ctl = e->ctl;
     Epi ep = epi;
+    if (e->pend_nblk && !e->fold_reduce)
+        if (int rc = flush_pending(e)) return rc;
+    Ctl* ctl = e->ctl;                              // the state this launch starts from
+    if (e->pend_nblk) {                             // this sweep reduces the previous one's partials itself
+        ep.pend_part = e->pend_part;
+        ep.pend_nblk = e->pend_nblk;
+        ep.pend_nq = partials_of(e->pend_sweep);
+        ep.pend_k = e->pend_epi.k;
+        ep.pend_role = e->pend_epi.role;
+        ep.pend_next = e->pend_epi.next;
+        ep.pend_sweep = e->pend_sweep;
+        e->cur_ctl ^= 1;
+        e->ctl = e->ctl_base + e->cur_ctl;
+        ep.ctl_out = e->ctl;
+        e->pend_nblk = 0;
+    }
+    double* part = e->part_buf[e->cur_part];
     void* args[] = {&sp, &mem, &part, &ctl, &ep};
     hipEvent_t a = nullptr, b = nullptr;
     if (e->profile) {
@@ -311,12 +410,14 @@ int launch_sweep(gs_engine* e, int k, int sweep, const Epi& epi) {
         e->pev.push_back(b);
         e->pev_tag.push_back(k * 8 + sweep);
     }
-    if (grid > kFenceBlocks) {          // the blocks only stored their partials: reduce and run the chain
-        const double* cpart = part;
-        unsigned nblk = unsigned(grid);
-        int sw = sweep;
-        void* rargs[] = {&cpart, &nblk, &ctl, &ep, &sw};
-        HIP_TRY(hipLaunchKernel(reduce_kernel_ptr(sweep), dim3(1), dim3(kThreads), rargs, 0, e->stream));
+    if (grid > kFenceBlocks) {          // the blocks only stored their partials: the next sweep reduces them
+        e->pend_part = part;
+        e->pend_nblk = unsigned(grid);
+        e->pend_sweep = sweep;
+        e->pend_epi = ep;
+        e->pend_epi.pend_nblk = 0;
+        e->cur_part ^= 1;
+        if (!e->fold_reduce) return flush_pending(e);
     }
     return 0;
 }
@@ -337,6 +438,7 @@ int collect_profile(gs_engine* e) {
 }
 
 int launch_small(gs_engine* e, void* fn, Epi epi, int k) {
+    if (int rc = flush_pending(e)) return rc;
     Ctl* ctl = e->ctl;
     if (k >= 0) {
         void* args[] = {&ctl, &epi, &k};
@@ -371,6 +473,7 @@ int run_from(const gs_engine* e, int k, int k1) {
 // A run of diagonal steps: pattern sums (one pass), their reduction, the run's decisions and bookkeeping (one block),
 // the product pass; its stored norm goes to the chain state.
 int launch_run(gs_engine* e, int ri, const Epi& base) {
+    if (int rc = flush_pending(e)) return rc;
     const RStep& rs = e->hrun[size_t(ri)];
     const RStep* rp = e->drun + ri;
     void* mem = e->arena;
@@ -475,6 +578,7 @@ int enqueue_run(gs_engine* e, int mode, int k0, int k1, const int8_t* forced_h, 
             continue;
         }
         if (ke - k >= 2) {
+            if (int rc = flush_pending(e)) return rc;
             const KStep* sp = e->dplain;
             void* mem = e->arena;
             Ctl* ctl = e->ctl;
@@ -521,7 +625,7 @@ int enqueue_run(gs_engine* e, int mode, int k0, int k1, const int8_t* forced_h, 
         if (int rc = launch_sweep(e, k, sweep, ep)) return rc;
         decided = ep.next != kNextSweep;
     }
-    return 0;
+    return flush_pending(e);
 }
 
 // ---- GF(2) helpers over u64 vectors (host)
@@ -1022,9 +1126,9 @@ void free_engine(gs_engine* e) {
     (void)hipFree(e->dplain);
     (void)hipFree(e->dfused);
     (void)hipFree(e->dlogical);
-    (void)hipFree(e->part);
+    (void)hipFree(e->part_buf[0]);
     (void)hipFree(e->counter);
-    (void)hipFree(e->ctl);
+    (void)hipFree(e->ctl_base);
     (void)hipFree(e->d_forced);
     (void)hipFree(e->d_U);
     (void)hipFree(e->d_bits);
@@ -1205,9 +1309,9 @@ int gs_engine_create(const gs_plan* plan, int storage, int device, gs_engine** o
     const size_t nsz = size_t(std::max(ns, 1));
     bool ok = dev_alloc((void**)&e->dplain, sizeof(KStep) * nsz) && dev_alloc((void**)&e->dfused, sizeof(KStep) * nsz) &&
               dev_alloc((void**)&e->dlogical, sizeof(KStep) * nsz) &&
-              dev_alloc((void**)&e->part, sizeof(double) * kPartDoubles) &&
+              dev_alloc((void**)&e->part_buf[0], 2 * sizeof(double) * kPartDoubles) &&
               dev_alloc((void**)&e->counter, sizeof(unsigned) * kCounters) &&
-              dev_alloc((void**)&e->ctl, sizeof(Ctl)) && dev_alloc((void**)&e->d_forced, C * nsz) &&
+              dev_alloc((void**)&e->ctl_base, 2 * sizeof(Ctl)) && dev_alloc((void**)&e->d_forced, C * nsz) &&
               dev_alloc((void**)&e->d_U, sizeof(double) * C * nsz) && dev_alloc((void**)&e->d_bits, C * nsz) &&
               dev_alloc((void**)&e->d_cond, sizeof(double) * C * nsz) &&
               dev_alloc((void**)&e->d_out, sizeof(double) * C * 4) && dev_alloc((void**)&e->d_amp, 16 * C) &&
@@ -1215,6 +1319,9 @@ int gs_engine_create(const gs_plan* plan, int storage, int device, gs_engine** o
               host_alloc((void**)&e->h_bits, C * nsz) && host_alloc((void**)&e->h_cond, sizeof(double) * C * nsz) &&
               host_alloc((void**)&e->h_out, sizeof(double) * C * 4) && host_alloc((void**)&e->h_amp, 16 * C);
     if (!ok) return bail(fail(11, "allocation of engine buffers failed"));
+    e->part_buf[1] = e->part_buf[0] + kPartDoubles;
+    e->part = e->part_buf[0];
+    e->ctl = e->ctl_base;
     if (hipStreamCreate(&e->stream) != hipSuccess || hipEventCreate(&e->ev0) != hipSuccess ||
         hipEventCreate(&e->ev1) != hipSuccess)
         return bail(fail(17, "stream/event creation failed"));
@@ -1673,6 +1780,55 @@ int gs_step_run(const gs_engine* eng, int k, int* out) {
     return 0;
 }
 
+// Each sweep reduces the block partials the previous sweep left (on), or a one-block kernel reduces them between the
+// two launches (off, default: every block repeating the reduction and chain step costs more than the launch it saves);
+// results are bitwise the same either way.
+int gs_set_fold_reduce(gs_engine* eng, int on) {
+    if (!eng) return fail(16, "null engine");
+    eng->fold_reduce = on != 0;
+    return 0;
+}
+
+// Race check of step k's layout on the device (debug mode; overwrites the arena, which serves as the owner map):
+// layout 0 the plain sweeps' coordinates, 1 the fused sweep's (the step must fold the next step's norms), 2 the plan's
+// own coordinates, 3 the run of diagonal steps starting at k (its passes' slots against the step's own table).
+// counts[4]: slots read by two cosets, slots written twice (visited twice by a run's passes), writes onto another
+// coset's input (run slots outside the step's table), slots checked.
+int gs_race_check(gs_engine* eng, int k, int layout, int64_t* counts) {
+    if (!eng || k < 0 || k >= eng->plan.ns || layout < 0 || layout > 3 || !counts) return fail(16, "bad argument");
+    gs_engine* e = eng;
+    if (layout == 1 && !e->fusable[size_t(k)]) return fail(16, "step " + std::to_string(k) + " has no fused layout");
+    const int ri = layout == 3 ? e->run_at[size_t(k)] : -1;
+    if (layout == 3 && ri < 0) return fail(16, "no run of diagonal steps starts at step " + std::to_string(k));
+    HIP_TRY(hipSetDevice(e->device));
+    const KStep* sp = (layout == 1 ? e->dfused : layout == 2 ? e->dlogical : e->dplain) + k;
+    const RStep* rp = ri >= 0 ? e->drun + ri : nullptr;
+    unsigned long long* own = static_cast<unsigned long long*>(e->arena);
+    const int stride = e->storage == GS_C64 ? 1 : 2;
+    void* dcnt = nullptr;
+    HIP_TRY(hipMalloc(&dcnt, 4 * sizeof(unsigned long long)));
+    unsigned long long h[4] = {0, 0, 0, 0};
+    int rc = 0;
+    do {
+        if (hipMemcpy(dcnt, h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) { rc = fail(17, "race check upload"); break; }
+        unsigned long long* cnt = static_cast<unsigned long long*>(dcnt);
+        const int modes[3] = {0, 1, layout == 3 ? 3 : 2};
+        for (int m : modes) {
+            const KStep* msp = sp;
+            if (m < 3 && layout == 3) msp = e->dplain + k;   // a run is checked against the step's plain table
+            hipLaunchKernelGGL(race_kernel, dim3(4096), dim3(256), 0, e->stream, msp, rp, own, stride, m, cnt);
+        }
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(e->stream) != hipSuccess ||
+            hipMemcpy(h, dcnt, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = fail(17, "race check failed");
+            break;
+        }
+    } while (false);
+    (void)hipFree(dcnt);
+    for (int i = 0; i < 4; i++) counts[i] = int64_t(h[i]);
+    return rc;
+}
+
 // Turn the folding of the next step's norms into the previous write off or on (for measurements; default on).
 int gs_set_fusion(gs_engine* eng, int on) {
     if (!eng) return fail(16, "null engine");
@@ -1744,8 +1900,8 @@ extern "C" int gs_prob_noalloc(const gs_plan* plan, int storage, int device, con
         used += (std::max<size_t>(n, 16) + 255) & ~size_t(255);
         return at;
     };
-    const size_t o_ctl = take(sizeof(Ctl)), o_counter = take(sizeof(unsigned) * kCounters),
-                 o_part = take(sizeof(double) * kPartDoubles), o_out = take(sizeof(double) * C * 4),
+    const size_t o_ctl = take(2 * sizeof(Ctl)), o_counter = take(sizeof(unsigned) * kCounters),
+                 o_part = take(2 * sizeof(double) * kPartDoubles), o_out = take(sizeof(double) * C * 4),
                  o_amp = take(16 * C), o_forced = take(C * nsz), o_U = take(sizeof(double) * C * nsz),
                  o_bits = take(C * nsz), o_cond = take(sizeof(double) * C * nsz), o_plain = take(sizeof(KStep) * nsz),
                  o_fused = take(sizeof(KStep) * nsz), o_arena = take((size_t(1) << e.plan.R) * e.esz);
@@ -1755,9 +1911,10 @@ extern "C" int gs_prob_noalloc(const gs_plan* plan, int storage, int device, con
     unsigned char* base = nullptr;
     if (int rc = scratch_address(device, &base)) return rc;
     if (int rc = build_steps(&e)) return rc;
-    e.ctl = reinterpret_cast<Ctl*>(base + o_ctl);
+    e.ctl_base = e.ctl = reinterpret_cast<Ctl*>(base + o_ctl);
     e.counter = reinterpret_cast<unsigned*>(base + o_counter);
-    e.part = reinterpret_cast<double*>(base + o_part);
+    e.part = e.part_buf[0] = reinterpret_cast<double*>(base + o_part);
+    e.part_buf[1] = e.part_buf[0] + kPartDoubles;
     e.d_out = reinterpret_cast<double*>(base + o_out);
     e.d_amp = base + o_amp;
     e.d_forced = reinterpret_cast<int8_t*>(base + o_forced);

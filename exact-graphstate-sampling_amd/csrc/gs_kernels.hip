@@ -341,6 +341,7 @@ __device__ __forceinline__ double2 step_m(const KStep& s, int b, int x) {
 
 __device__ void publish(Ctl& c, const Epi& e) {
     if (e.mode == kChain1 && !c.dead) c.logp = log2(c.prev) - e.ns;
+    if (e.quiet) return;
     e.out[0] = c.logp;
     e.out[1] = c.logscale;
     e.out[2] = c.prev;
@@ -351,26 +352,29 @@ __device__ void publish(Ctl& c, const Epi& e) {
 __device__ void finish_write(Ctl& c, const Epi& e, int k, double nb, double nst) {
     c.nstored = nst;
     if (e.mode == kChain1) {                                   // gsamp2.cpp:67
-        e.cond[k] = nb / (2 * c.prev);
+        if (!e.quiet) e.cond[k] = nb / (2 * c.prev);
         c.prev = nb;
         if (nb <= 0) {
-            for (int q = k; q < e.ns; q++) e.cond[q] = 0;
+            if (!e.quiet)
+                for (int q = k; q < e.ns; q++) e.cond[q] = 0;
             c.logp = -INFINITY;
             c.dead = 1;
         }
     } else if (e.mode == kRun) {                               // gsamp2.cpp:93-95
         c.logscale += 0.5 * log2(c.prev);
-        e.cond[k] = nb / 2;
+        const double cd = nb / 2;
+        if (!e.quiet) e.cond[k] = cd;
         c.prev = nb;
         if (nb <= 0) {
-            for (int q = k; q < e.k1; q++) {
-                e.cond[q] = 0;
-                e.bits[q] = e.forced[q] < 0 ? 0 : e.forced[q];
-            }
+            if (!e.quiet)
+                for (int q = k; q < e.k1; q++) {
+                    e.cond[q] = 0;
+                    e.bits[q] = e.forced[q] < 0 ? 0 : e.forced[q];
+                }
             c.logp = -INFINITY;
             c.dead = 1;
         } else {
-            c.logp += log2(e.cond[k]);
+            c.logp += log2(cd);
         }
     }
     if (c.dead) publish(c, e);
@@ -417,7 +421,7 @@ __device__ void decide(Ctl& c, const Epi& e, int k, int norm, const double* red)
         sc = 1.0 / sqrt(c.prev);
         if (bv < 0) bv = e.U[c.qidx++] < n1 / (n0 + n1);
     }
-    e.bits[k] = int8_t(bv);
+    if (!e.quiet) e.bits[k] = int8_t(bv);
     c.bv = bv;
     c.sc = sc;
     const double2 m0 = step_m(s, bv, 0), m1 = step_m(s, bv, 1);
@@ -427,18 +431,28 @@ __device__ void decide(Ctl& c, const Epi& e, int k, int norm, const double* red)
     c.w[3] = m1.y * sc;
 }
 
-// Block-wide sum of v[NQ] over the NT threads in a fixed tree order; the result is in sh[0][*].
+// Block-wide sum of v[NQ] over the NT threads in a fixed order (a butterfly inside each 64-lane wave, then the wave
+// sums in wave order by one thread: two barriers instead of a log2(NT)-level tree); the result is in sh[0][*].
 template <int NQ, int NT = kThreads>
-__device__ __forceinline__ void block_tree_sum(double (&sh)[NT][NQ], const double (&v)[NQ], int t) {
+__device__ __forceinline__ void block_tree_sum(double (&sh)[NT / 64][NQ], const double (&v)[NQ], int t) {
+    double r[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; q++) sh[t][q] = v[q];
-    __syncthreads();
-    for (int w = NT / 2; w > 0; w >>= 1) {
-        if (t < w)
-#pragma unroll
-            for (int q = 0; q < NQ; q++) sh[t][q] += sh[t + w][q];
-        __syncthreads();
+    for (int q = 0; q < NQ; q++) {
+        r[q] = v[q];
+        for (int o = 32; o > 0; o >>= 1) r[q] += __shfl_xor(r[q], o, 64);
     }
+    if ((t & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) sh[t >> 6][q] = r[q];
+    __syncthreads();
+    if (t == 0)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            double x = sh[0][q];
+            for (int w = 1; w < NT / 64; w++) x += sh[w][q];
+            sh[0][q] = x;
+        }
+    __syncthreads();
 }
 
 // Chain logic after a sweep, from its reduced sums (one thread).
@@ -463,8 +477,8 @@ __device__ void chain_step(const double (&red)[NQ], Ctl* ctl, const Epi& e, int 
 
 // Fixed-order sum of the first NQ values of nblk block partials by one block; the result is in sh[0][*].
 template <int NQ>
-__device__ __forceinline__ void reduce_partials(double (&sh)[kThreads][NQ], const double* part, unsigned nblk, int t,
-                                                bool atomic_loads) {
+__device__ __forceinline__ void reduce_partials(double (&sh)[kThreads / 64][NQ], const double* part, unsigned nblk,
+                                                int t, bool atomic_loads) {
     double r[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) r[q] = 0;
@@ -487,7 +501,7 @@ __device__ void sweep_epilogue(double (&acc)[NQ], double* part, Ctl* ctl, const 
 #pragma unroll
     for (int q = 0; q < NQ; q++)
         for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
-    __shared__ double sh[kThreads][NQ];
+    __shared__ double sh[kThreads / 64][NQ];
     __shared__ int s_last;
     const int t = threadIdx.x;
     const unsigned nblk = gridDim.x;
@@ -529,7 +543,7 @@ template <int NQ>
 __global__ __launch_bounds__(kThreads) void sweep_reduce_kernel(const double* __restrict__ part, unsigned nblk,
                                                                 Ctl* __restrict__ ctl, Epi e, int sweep) {
     if (ctl->dead) return;
-    __shared__ double sh[kThreads][NQ];
+    __shared__ double sh[kThreads / 64][NQ];
     const int t = threadIdx.x;
     reduce_partials<NQ>(sh, part, nblk, t, false);
     if (t != 0) return;
@@ -537,6 +551,41 @@ __global__ __launch_bounds__(kThreads) void sweep_reduce_kernel(const double* __
 #pragma unroll
     for (int q = 0; q < NQ; q++) red[q] = sh[0][q];
     chain_step<NQ>(red, ctl, e, sweep);
+}
+
+// The chain step of block partials a previous sweep left unreduced, run at the start of the next sweep by every block
+// (fixed-order sums: every block computes the same state); thread 0 of block 0 alone writes the string's outputs and
+// stores the state at epi.ctl_out.  cl receives the state in every thread.
+template <int NQ>
+__device__ void pending_reduce_step(Ctl& cl, const Ctl* ctl, const Epi& epi, Ctl* shc) {
+    __shared__ double sh[kThreads / 64][NQ];
+    const int t = threadIdx.x;
+    reduce_partials<NQ>(sh, epi.pend_part, epi.pend_nblk, t, false);
+    if (t == 0) {
+        Ctl c = *ctl;
+        if (!c.dead) {
+            double red[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) red[q] = sh[0][q];
+            Epi ep = epi;
+            ep.k = epi.pend_k;
+            ep.role = epi.pend_role;
+            ep.next = epi.pend_next;
+            ep.quiet = blockIdx.x != 0;
+            chain_step<NQ>(red, &c, ep, epi.pend_sweep);
+        }
+        *shc = c;
+        if (blockIdx.x == 0) *epi.ctl_out = c;
+    }
+    __syncthreads();
+    cl = *shc;
+}
+
+__device__ void pending_step(Ctl& cl, const Ctl* ctl, const Epi& epi) {
+    __shared__ Ctl shc;
+    if (epi.pend_nq == 2) pending_reduce_step<2>(cl, ctl, epi, &shc);
+    else if (epi.pend_nq == 4) pending_reduce_step<4>(cl, ctl, epi, &shc);
+    else pending_reduce_step<6>(cl, ctl, epi, &shc);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -555,7 +604,6 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     constexpr bool GRAM1 = SWEEP == kWriteGram1;
     constexpr int NA = SWEEP == kGram ? 4 : SWEEP == kDirect ? 2 : SWEEP == kWrite ? 2
                      : SWEEP == kWriteGram || SWEEP == kWriteGram1 ? 6 : 4;
-    if (ctl->dead) return;
     const KStep& s = *sp;
     const int nk = s.nk;
     const int tid = threadIdx.x;
@@ -605,11 +653,6 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     double2 M[2][2] = {{w0, w0}, {w0, w0}};
     double2 w1t[MO];
     double rho[NTQ];
-    if constexpr (WRITES) {
-        w0 = make_double2(ctl->w[0], ctl->w[1]);
-        w1 = make_double2(ctl->w[2], ctl->w[3]);
-        write_coefs<ROUTE, NTQ, MO>(w1, s.parmask, w1t, rho);
-    }
     if constexpr (SWEEP == kDirect || SWEEP == kWriteDirect) {
         const KStep& ms = SWEEP == kDirect ? epi.plain[epi.k] : epi.plain[epi.k + 1];
 #pragma unroll
@@ -655,6 +698,28 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
         a_it = s.cols[jit + s.lg_iter - 1];
         p_it = pb(jit + s.lg_iter - 1);
     }
+    // Chain state of this launch: the kernel's ctl, or -- when the previous sweep left its block partials unreduced --
+    // the chain step of those partials, run by every block on its own copy (the same fixed-order sums, so the same
+    // result everywhere) while the first tile's loads are in flight; block 0 stores it at ctl_out.
+    bool first_loaded = false;
+    Ctl* cur = ctl;
+    Ctl cl;
+    if (epi.pend_nblk) {
+        cur = epi.ctl_out;
+        if (active) {
+            load_tile(a_tb ^ a_it, e);
+            first_loaded = true;
+        }
+        pending_step(cl, ctl, epi);
+    } else {
+        cl = *ctl;
+    }
+    if (cl.dead) return;
+    if constexpr (WRITES) {
+        w0 = make_double2(cl.w[0], cl.w[1]);
+        w1 = make_double2(cl.w[2], cl.w[3]);
+        write_coefs<ROUTE, NTQ, MO>(w1, s.parmask, w1t, rho);
+    }
     for (int it = 0; it < niter; it++) {
         if (it > 0) {                               // Gray code: one coordinate changes
             const int b = __builtin_ctz(unsigned(rev ? niter - it : it));
@@ -664,7 +729,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
         const uint64_t ag = a_tb ^ a_it;
         const int pbase = p_thr ^ p_blk ^ p_it;
         if (!active) continue;
-        load_tile(ag, e);
+        if (it > 0 || !first_loaded) load_tile(ag, e);
 #pragma unroll
         for (int j = 0; j < NTQ; j++)
 #pragma unroll
@@ -752,7 +817,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     }
     if constexpr (SWEEP == kWriteGram) gram_assemble<ROUTE2>(gi, s.par2, acc + 2);
     if constexpr (GRAM1) acc[3] = acc[2];
-    sweep_epilogue<NA>(acc, part, ctl, epi, GRAM1 ? int(kWriteGram) : SWEEP);
+    sweep_epilogue<NA>(acc, part, cur, epi, GRAM1 ? int(kWriteGram) : SWEEP);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -831,7 +896,7 @@ template <class S>
 __global__ __launch_bounds__(kSmallThreads) void small_run_kernel(const KStep* __restrict__ steps, S* __restrict__ mem,
                                                                   Ctl* __restrict__ ctl, Epi e, int k0, int k1,
                                                                   int decided0) {
-    __shared__ double sh[kSmallThreads][4];
+    __shared__ double sh[kSmallThreads / 64][4];
     __shared__ int s_dead;
     const int t = threadIdx.x;
     Ctl& c = *ctl;
@@ -963,7 +1028,7 @@ __global__ __launch_bounds__(kRunBins) void run_decide_kernel(const RStep* __res
     if (c.dead) return;
     const RStep& r = *rp;
     const int t = threadIdx.x;
-    __shared__ double sh[kRunBins][2];
+    __shared__ double sh[kRunBins / 64][2];
     __shared__ int s_dead;
     double H = 0;
     for (unsigned g = 0; g < G; g++) H += chunk[size_t(g) * kRunBins + t];

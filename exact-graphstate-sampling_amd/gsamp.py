@@ -21,7 +21,7 @@ SEED_PLAIN, SEED_SEQ = 0, 1
 ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy", "gs_engine_create",
                "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
                "gs_last_device_ms", "gs_last_error", "gs_profile_enable", "gs_profile_read",
-               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_grid_log2_one_slot", "gs_set_small_steps", "gs_set_walk", "gs_set_xcd_blocks", "gs_set_plain_pairs", "gs_set_fused_run", "gs_set_diag_runs", "gs_step_run", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
+               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_grid_log2_one_slot", "gs_set_small_steps", "gs_set_walk", "gs_set_xcd_blocks", "gs_set_plain_pairs", "gs_set_fused_run", "gs_set_diag_runs", "gs_step_run", "gs_set_fold_reduce", "gs_race_check", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
                "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_set_xcd_items", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms", "gs_tt_plan_load", "gs_tt_plan_info",
                "gs_tt_plan_destroy", "gs_tt_engine_create", "gs_tt_engine_destroy", "gs_tt_prob", "gs_tt_chain",
@@ -82,6 +82,8 @@ def library():
         "gs_set_plain_pairs": (I, [P, I, I]),
         "gs_set_fused_run": (I, [P, I]),
         "gs_set_diag_runs": (I, [P, I, I]),
+        "gs_set_fold_reduce": (I, [P, I]),
+        "gs_race_check": (I, [P, I, I, ctypes.POINTER(ctypes.c_int64)]),
         "gs_step_run": (I, [P, I, ip]),
         "gs_copy_probe": (I, [I, I64, dp]),
         "gs_batch_create": (I, [ctypes.POINTER(P), ip, I, I, I, pp]),
@@ -334,6 +336,39 @@ class Engine:
         out = np.zeros(4, np.int32)
         _check(library().gs_step_run(self._h, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         return None if out[0] < 0 else tuple(int(x) for x in out)
+
+    RACE_LAYOUTS = ("plain", "fused", "logical", "run")
+
+    def race_check(self, k, layout="plain"):
+        """Device race check of step k's layout (overwrites the arena): dict(read_twice, written_twice,
+        clobbered, checked); for a run of diagonal steps written_twice counts slots visited twice and clobbered the
+        slots outside the step's table."""
+        out = np.zeros(4, np.int64)
+        _check(library().gs_race_check(self._h, k, self.RACE_LAYOUTS.index(layout),
+                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return dict(zip(("read_twice", "written_twice", "clobbered", "checked"), (int(x) for x in out)))
+
+    def race_check_all(self):
+        """Race check of every layout the kernels walk (plain, fused where a step folds the next norms, runs of
+        diagonal steps); returns the summed counts, and raises AssertionError on any conflict."""
+        tot = dict(read_twice=0, written_twice=0, clobbered=0, checked=0)
+        for k in range(self.ns):
+            lays = ["plain"] + (["fused"] if self.geometry(k)["fusable"] else [])
+            run = self.run_of(k)
+            if run is not None and run[0] == k:
+                lays.append("run")
+            for lay in lays:
+                r = self.race_check(k, lay)
+                for key in tot:
+                    tot[key] += r[key]
+                if r["read_twice"] or r["written_twice"] or r["clobbered"]:
+                    raise AssertionError(f"step {k} layout {lay}: {r}")
+        return tot
+
+    def set_fold_reduce(self, on):
+        """Each sweep reduces the partial sums the previous sweep left (True) or a one-block kernel does in between
+        (False, default); bitwise the same results."""
+        _check(library().gs_set_fold_reduce(self._h, 1 if on else 0))
 
     def set_fusion(self, on):
         """Fold each sampled step's norm pass into the previous write sweep where possible (default on)."""

@@ -197,8 +197,18 @@ int gs_set_plain_pairs(gs_engine* eng, int ntq_max, int far);
 int gs_set_diag_runs(gs_engine* eng, int max_steps, int min_nk);
 /* The run of diagonal steps holding step k, out[4]: first step, steps, log2 blocks, log2 tiles per block (-1: none). */
 int gs_step_run(const gs_engine* eng, int k, int* out);
+/* A sweep whose blocks leave their partial sums unreduced has them reduced by every block of the next sweep, which
+   then needs no separate reduction launch (on = 1), or by a one-block kernel between the two (0, default); the
+   results are bitwise the same. */
+int gs_set_fold_reduce(gs_engine* eng, int on);
 /* Runs of two or more steps with at most 2^nk cosets go to one single-block launch (nk -1: never; default 10). */
 int gs_set_small_steps(gs_engine* eng, int nk);
+/* Device race check of step k (debug mode; overwrites the arena, used as an owner map of the slots): layout 0 the
+   plain sweeps' coset coordinates, 1 the fused sweep's, 2 the plan's own, 3 the run of diagonal steps starting at k.
+   counts[4]: slots read by two cosets, slots written twice (for a run: visited twice), writes onto a slot another
+   coset reads (for a run: slots outside the step's table), slots checked -- the reference's in-place asserts
+   (inplace.py:126-137) on the layouts the kernels actually walk. */
+int gs_race_check(gs_engine* eng, int k, int layout, int64_t* counts);
 /* One plain device copy of `bytes` (read + write), *ms its duration: calibrates hardware byte counters. */
 int gs_copy_probe(int device, int64_t bytes, double* ms);
 

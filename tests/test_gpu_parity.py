@@ -468,3 +468,24 @@ def test_diag_runs_vs_oracle(gadget_plan, tmp_path, storage, grid_log2):
     assert (dnk, dntq) == (nk, ntq) and abs(dls - ls) <= max(tol, 1e-12)
     assert np.max(np.abs(np.asarray(dump) - tab)) <= 10 * tol * max(1.0, np.max(np.abs(tab)))
     assert np.array_equal(ob0, ob) and close(lph0, lph, 0) <= tol
+
+
+@pytest.mark.parametrize("storage", [gs.C64, gs.C128])
+def test_fold_reduce_bitwise_equal(golden, storage):
+    """IBM #151 width-30 plan: sweeps that reduce the previous sweep's block partials themselves against a one-block
+    reduction kernel between them (default) -- sample, prob, chain and hybrid outputs bitwise equal (the same
+    fixed-order sums and chain step either way)."""
+    d = os.path.join(golden, "ibm")
+    strings = egs.planio.read_strings(os.path.join(d, "gate_strings.txt"), 70)
+    hy = strings.copy()
+    hy[:, 50:] = -1
+    got = {}
+    print(f"fold reduce on/off, IBM width 30, storage={TAG[storage]}", flush=True)
+    with gs.Engine(os.path.join(d, "nq70_inplace_plan.txt"), storage) as e:
+        for fold in (True, False):
+            e.set_fold_reduce(fold)
+            got[fold] = (e.sample(2, 2000000), e.prob(strings), e.chain(strings), e.hybrid(hy, 17))
+    (b1, l1), p1, (c1, q1, a1), (h1, hl1, hc1) = got[True]
+    (b0, l0), p0, (c0, q0, a0), (h0, hl0, hc0) = got[False]
+    for x, y in ((b1, b0), (l1, l0), (p1, p0), (c1, c0), (q1, q0), (a1, a0), (h1, h0), (hl1, hl0), (hc1, hc0)):
+        assert np.array_equal(x, y)
