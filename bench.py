@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--fold-reduce", action="store_true",
                     help="reduce each sweep's block partials in the next sweep instead of in a one-block kernel")
     ap.add_argument("--grid-log2", type=int, default=0, help="cap sweeps at 2^N blocks (default: engine default)")
+    ap.add_argument("--small-steps", type=int, default=None, metavar="NK",
+                    help="runs of steps with at most 2^NK cosets go to one single-block launch (-1: never; default: "
+                         "engine default)")
     ap.add_argument("--legs", default="c3,c4,c5",
                     help="short legs of the other workloads appended to the headline line (comma list; '' for none)")
     ap.add_argument("--cpu-reps", type=int, default=3, help="repetitions of each CPU baseline timing (median)")
@@ -115,6 +118,8 @@ def main():
             e.set_fold_reduce(True)
         if args.grid_log2:
             e.set_grid_log2(args.grid_log2)
+        if args.small_steps is not None:
+            e.set_small_steps(args.small_steps)
         if W:
             e.sample(W, args.seed + 10 ** 6)
         # timed region: K samples, no instrumentation.  Engine calls are synchronous (each returns after its stream
