@@ -567,6 +567,8 @@ def ibm_w30(gs, egs, K, args):
             e.set_fusion(False)
         if args.grid_log2:
             e.set_grid_log2(args.grid_log2)
+        if args.small_steps is not None:
+            e.set_small_steps(args.small_steps)
         e.sample(1, 7)
         t0 = time.perf_counter()
         bits, lp = e.sample(K, 2000000)

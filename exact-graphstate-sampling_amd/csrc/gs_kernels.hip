@@ -836,25 +836,49 @@ __global__ void publish_kernel(Ctl* ctl, Epi e) { publish(*ctl, e); }
 // bookkeeping -- with block barriers in place of launch boundaries.  The arithmetic per coset is the sweep kernels'
 // (FP64 routed slot sums, one rounding per store) and the chain functions are the same.
 
-constexpr int kSmallThreads = 1024;       // one block of 16 waves: enough loads in flight for 2^10-coset steps
+constexpr int kSmallThreads = 512;        // one block of 8 waves: 256 registers per lane, no spills (16 waves: 4.48
+                                          // against 4.41 ms per width-26 sample)
 
-// One pass of a small step by the block's threads (cosets tid, tid + 1024, ...): sums into acc (Gram: 4, direct: 2,
-// write: written norm and stored norm).
+// One pass of a small step by the block's threads (cosets tid, tid + kSmallThreads, ...): sums into acc (Gram: 4,
+// direct: 2, write: written norm and stored norm).  The low kSmallLog2 coordinates of a coset are the thread's, the
+// rest the iteration's.
+constexpr int kSmallLog2 = 9;
+static_assert(kSmallThreads == 1 << kSmallLog2, "one coset per thread and iteration");
 template <class S, int ROUTE, int NTQ>
-__device__ void small_pass(const KStep& s, S* mem, int sweep, const double2 (&M)[2][2], double2 w0, double2 w1,
-                           double (&acc)[4]) {
+__device__ __forceinline__ void small_pass(const KStep& s, S* mem, int sweep, const double2 (&M)[2][2], double2 w0,
+                                           double2 w1, double (&acc)[4]) {
     constexpr int MO = ROUTE == kRoute1 ? 1 : 2;
-    const uint64_t n = uint64_t(1) << s.nk;
-    for (uint64_t c = threadIdx.x; c < n; c += kSmallThreads) {
-        uint64_t base = 0;
-        for (int j = 0; j < s.nk; j++)
+    constexpr int kHi = 5;          // iteration coordinates whose columns are read up front (steps up to 2^14 cosets)
+    // the descriptor fields the pass needs, read once with independent loads
+    const int nk = s.nk;
+    uint64_t lo[kSmallLog2], hi[kHi], off[NTQ];
+#pragma unroll
+    for (int j = 0; j < kSmallLog2; j++) lo[j] = s.cols[j];
+#pragma unroll
+    for (int b = 0; b < kHi; b++) hi[b] = s.cols[kSmallLog2 + b];
+#pragma unroll
+    for (int j = 0; j < NTQ; j++) off[j] = s.off[j];
+    const uint64_t g = s.g, pm = s.pmask[0];
+    const uint32_t parmask = s.parmask;
+    const uint64_t n = uint64_t(1) << nk;
+    const unsigned t = threadIdx.x;
+    uint64_t bt = 0;
+#pragma unroll
+    for (int j = 0; j < kSmallLog2; j++)
+        if (j < nk && ((t >> j) & 1)) bt ^= lo[j];
+    for (uint64_t c = t; c < n; c += kSmallThreads) {
+        uint64_t base = bt;
+#pragma unroll
+        for (int b = 0; b < kHi; b++)
+            if (kSmallLog2 + b < nk && ((c >> (kSmallLog2 + b)) & 1)) base ^= hi[b];
+        for (int j = kSmallLog2 + kHi; j < nk; j++)
             if ((c >> j) & 1) base ^= s.cols[j];
-        const int pc = __popcll(c & s.pmask[0]) & 1;
+        const int pc = __popcll(c & pm) & 1;
         double2 wv[NTQ];
 #pragma unroll
-        for (int j = 0; j < NTQ; j++) wv[j] = Cx<S>::widen(mem[base ^ s.off[j]]);
+        for (int j = 0; j < NTQ; j++) wv[j] = Cx<S>::widen(mem[base ^ off[j]]);
         double2 A[MO][2];
-        route_terms<ROUTE, NTQ, MO>(A, wv, pc, s.parmask);
+        route_terms<ROUTE, NTQ, MO>(A, wv, pc, parmask);
         if (sweep == kGram) {
             gram_acc<MO>(A, acc);
         } else if (sweep == kDirect) {
@@ -864,7 +888,7 @@ __device__ void small_pass(const KStep& s, S* mem, int sweep, const double2 (&M)
             for (int t = 0; t < MO; t++) {
                 const double2 v = cfma(w0, A[t][0], cmul(w1, A[t][1]));
                 const S sv = Cx<S>::narrow(v);
-                mem[t ? base ^ s.g : base] = sv;
+                mem[t ? base ^ g : base] = sv;
                 add_norm(acc[0], v);
                 add_norm(acc[1], Cx<S>::widen(sv));
             }
@@ -872,8 +896,31 @@ __device__ void small_pass(const KStep& s, S* mem, int sweep, const double2 (&M)
     }
 }
 
+// Block-wide sum of v[4] for thread 0 alone (one barrier): a butterfly inside each wave, then thread 0 adds the waves
+// in order -- the order of block_tree_sum, without its second barrier.
+__device__ __forceinline__ void small_block_sum(double (&sh)[kSmallThreads / 64][4], const double (&v)[4], int t,
+                                                double (&out)[4]) {
+    double r[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        r[q] = v[q];
+        for (int o = 32; o > 0; o >>= 1) r[q] += __shfl_xor(r[q], o, 64);
+    }
+    if ((t & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < 4; q++) sh[t >> 6][q] = r[q];
+    __syncthreads();
+    if (t == 0)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            double x = sh[0][q];
+            for (int w = 1; w < kSmallThreads / 64; w++) x += sh[w][q];
+            out[q] = x;
+        }
+}
+
 template <class S>
-__device__ void small_pass_any(const KStep& s, S* mem, int sweep, const double2 (&M)[2][2], double2 w0, double2 w1,
+__device__ __forceinline__ void small_pass_any(const KStep& s, S* mem, int sweep, const double2 (&M)[2][2], double2 w0, double2 w1,
                                double (&acc)[4]) {
     switch (s.ntq * 4 + s.route) {
         case 1 * 4 + kRoute1: small_pass<S, kRoute1, 1>(s, mem, sweep, M, w0, w1, acc); break;
@@ -898,9 +945,12 @@ __global__ __launch_bounds__(kSmallThreads) void small_run_kernel(const KStep* _
                                                                   int decided0) {
     __shared__ double sh[kSmallThreads / 64][4];
     __shared__ int s_dead;
+    __shared__ Ctl c;               // the chain state, in LDS while the run lasts
     const int t = threadIdx.x;
-    Ctl& c = *ctl;
-    if (t == 0) s_dead = c.dead;
+    if (t == 0) {
+        c = *ctl;
+        s_dead = c.dead;
+    }
     __syncthreads();
     const bool fstr = e.forced != nullptr && e.mode == kChain;
     for (int k = k0; k < k1 && !s_dead; k++) {
@@ -916,11 +966,10 @@ __global__ __launch_bounds__(kSmallThreads) void small_run_kernel(const KStep* _
             int nm = kGram;
             if (e.mode == kChain1 || (e.mode == kRun && fb >= 0) || s.growth) nm = -1;
             else if (fstr) nm = kDirect;
-            double acc[4] = {0, 0, 0, 0};
+            double acc[4] = {0, 0, 0, 0}, red[4];
             if (nm >= 0) small_pass_any<S>(s, mem, nm, M, zero, zero, acc);
-            block_tree_sum<4, kSmallThreads>(sh, acc, t);
+            small_block_sum(sh, acc, t, red);
             if (t == 0) {
-                const double red[4] = {sh[0][0], sh[0][1], sh[0][2], sh[0][3]};
                 decide(c, e, k, nm, nm >= 0 ? red : nullptr);
                 s_dead = c.dead;
             }
@@ -928,16 +977,17 @@ __global__ __launch_bounds__(kSmallThreads) void small_run_kernel(const KStep* _
             if (s_dead) break;
         }
         const double2 w0 = make_double2(c.w[0], c.w[1]), w1 = make_double2(c.w[2], c.w[3]);
-        double acc[4] = {0, 0, 0, 0};
+        double acc[4] = {0, 0, 0, 0}, red[4];
         small_pass_any<S>(s, mem, kWrite, M, w0, w1, acc);
-        block_tree_sum<4, kSmallThreads>(sh, acc, t);
+        small_block_sum(sh, acc, t, red);
         if (t == 0) {
-            finish_write(c, e, k, sh[0][0], sh[0][1]);
+            finish_write(c, e, k, red[0], red[1]);
             if (!c.dead && k + 1 == e.k1) publish(c, e);
             s_dead = c.dead;
         }
         __syncthreads();
     }
+    if (t == 0) *ctl = c;
 }
 
 // ---------------------------------------------------------------------------------------------------------------
