@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--c5-strings", type=int, default=256)
     ap.add_argument("--c5-c128", type=int, default=4, help="c5 samples/strings re-scored in a complex128 arena")
     ap.add_argument("--no-fusion", action="store_true", help="run every norm pass as its own sweep")
+    ap.add_argument("--no-fused4", action="store_true",
+                    help="norm passes of four-slot steps as their own sweeps (the rest stay folded)")
     ap.add_argument("--fold-reduce", action="store_true",
                     help="reduce each sweep's block partials in the next sweep instead of in a one-block kernel")
     ap.add_argument("--grid-log2", type=int, default=0, help="cap sweeps at 2^N blocks (default: engine default)")
@@ -114,6 +116,8 @@ def main():
     with gs.Engine(plan_path, gs.C64) as e:
         if args.no_fusion:
             e.set_fusion(False)
+        if args.no_fused4:
+            e.set_fused4(False)
         if args.fold_reduce:
             e.set_fold_reduce(True)
         if args.grid_log2:
@@ -565,6 +569,8 @@ def ibm_w30(gs, egs, K, args):
     with gs.Engine(path, gs.C64) as e:
         if args.no_fusion:
             e.set_fusion(False)
+        if args.no_fused4:
+            e.set_fused4(False)
         if args.grid_log2:
             e.set_grid_log2(args.grid_log2)
         if args.small_steps is not None:

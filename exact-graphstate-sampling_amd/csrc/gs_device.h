@@ -39,15 +39,17 @@ struct KStep {
     uint64_t cols[40];     // physical column of each coset coordinate
     uint64_t off[4];       // coset offsets, relabelled so that the routing is canonical
     uint64_t g;            // second output slot offset (m = 2)
-    uint64_t pmask[3];     // parity masks over the coordinates: [0] output sign (rhoc), [1] next step's slot index,
-                           // [2] next step's sign (fused sweeps)
+    uint64_t pmask[4];     // parity masks over the coordinates: [0] output sign (rhoc), [1] next step's slot index,
+                           // [2] next step's sign (fused sweeps), [3] next step's second slot-index bit (fused
+                           // sweeps of a step followed by a four-slot step)
     double mre[2][2];      // M[b][x]
     double mim[2][2];
     uint32_t parmask;      // bit j: parity(qv_j & rho) of offset j
     int nk;                // number of coset coordinates
     int lr;                // log2 run length (cosets per thread along unit bits)
     int vg;                // 1, or 2 when coordinate lr is a per-thread coordinate too: the pairing coordinate of the
-                           // next step (fused sweeps), or a second run per thread (plain sweeps of 1- and 2-slot steps)
+                           // next step (fused sweeps), or a second run per thread (plain sweeps of 1- and 2-slot steps);
+                           // 4 when coordinates lr, lr+1 both pair the slots of a four-slot next step (fused sweeps)
     int lg_grid;           // log2 blocks
     int lg_iter;           // log2 tiles per block
     int ntq, m, route;
@@ -55,7 +57,9 @@ struct KStep {
     int route2;            // fused: next step's route
     int par2;              // fused: next step's par of its second offset
     int eps;               // fused: the partner of output tau is output tau ^ eps of the paired coset
-    int phig, nug;         // fused: slot-index and sign parity of g
+    int eps2;              // fused, four-slot next step: the same for the second pairing coordinate
+    int parmask2;          // fused, four-slot next step: its par bits of offsets 0..3
+    int phig, nug;         // fused: slot-index (two bits for a four-slot next step) and sign parity of g
     int rev;               // walk the tiles of each block in reverse Gray order
     int xcd;               // give each XCD a contiguous eighth of the block coordinates (grids of 8 blocks or more)
     int sgg, sgk;          // kWriteGram1: next step's sign parity of g, and its constant (pmask[2]: per coordinate)

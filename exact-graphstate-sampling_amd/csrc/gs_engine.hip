@@ -319,6 +319,7 @@ struct gs_engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
     bool allow_fusion = true;
+    bool allow_fused4 = true;       // fold a four-slot next step's norms into the write (two pairing coordinates)
     int grid_cap = kDefaultGridLog2;  // log2 of the most blocks a sweep launches
     int grid_cap1 = kDefaultGridLog2One;  // ... a plain sweep of a one-slot step
     int grid_cap1_big = kDefaultGridLog2OneBig;  // ... of a one-slot step with at least 2^kBigStepLog2 cosets
@@ -775,7 +776,7 @@ bool build_fused(const gs_engine* e, int k, KStep& f) {
     if (s0.m == 2 && !phinu(s0.g, phig, nug)) return false;
     f = p0;
     memset(f.cols, 0, sizeof f.cols);
-    f.pmask[0] = f.pmask[1] = f.pmask[2] = 0;
+    f.pmask[0] = f.pmask[1] = f.pmask[2] = f.pmask[3] = 0;
     for (int i = 0; i < nk; i++) {
         const uint64_t b = basis[size_t(i)];
         uint64_t col = 0;
@@ -800,6 +801,85 @@ bool build_fused(const gs_engine* e, int k, KStep& f) {
     f.phig = phig;
     f.nug = nug;
     set_geometry(f, lrf + 1, e->grid_cap);
+    return true;
+}
+
+// Fused descriptor of step k when step k+1 is a four-slot step (offsets 0, o1, o2, o1 ^ o2): coordinates lr and lr+1
+// become gamma1 and gamma2, the coset parts of o1 and o2 in the live space after step k (o_i = cols . gamma_i ^ eps_i
+// g), so a thread's cosets c, c ^ gamma1, c ^ gamma2, c ^ gamma1 ^ gamma2 hold all four slots of the next-step cosets
+// their outputs belong to: output tau of coset c ^ gamma(m) is slot j0 ^ m of the next coset holding output tau ^
+// eps(m) of c.  Slot-index bits and sign of the next step are parities of the stored address, as in build_fused.
+bool build_fused4(const gs_engine* e, int k, KStep& f) {
+    const Plan& P = e->plan;
+    if (k + 1 >= P.ns || !e->allow_fused4) return false;
+    const StepInfo& s0 = P.steps[size_t(k)];
+    const KStep& p0 = e->hplain[size_t(k)];
+    const KStep& p1 = e->hplain[size_t(k + 1)];
+    if (p1.ntq != 4 || p1.growth || p1.off[0] != 0 || p1.off[3] != (p1.off[1] ^ p1.off[2])) return false;
+    const int nk = s0.nk, lrf = e->fused_run >= 0 ? e->fused_run : fused_lr(e->storage);
+    std::vector<uint64_t> lead;
+    for (int i = 0; i < lrf; i++) {
+        uint64_t u = 0;
+        if (i >= nk || !gf2_solve(s0.cols, uint64_t(1) << i, u)) return false;
+        lead.push_back(u);
+    }
+    std::vector<uint64_t> live(s0.cols.begin(), s0.cols.end());
+    if (s0.m == 2) live.push_back(s0.g);
+    const uint64_t cmask = nk >= 64 ? ~uint64_t(0) : (uint64_t(1) << nk) - 1;
+    int epsv[2] = {0, 0};
+    for (int i = 0; i < 2; i++) {
+        uint64_t coef = 0;
+        if (!gf2_solve(live, p1.off[1 + i], coef)) return false;
+        lead.push_back(coef & cmask);
+        epsv[i] = s0.m == 2 ? int((coef >> nk) & 1) : 0;
+    }
+    std::vector<uint64_t> basis;
+    if (!coord_basis(s0.cols, lead, lrf, basis)) return false;
+    const StepInfo& s1 = P.steps[size_t(k + 1)];
+    std::vector<uint64_t> nb(s1.cols.begin(), s1.cols.end());
+    nb.push_back(p1.off[1]);
+    nb.push_back(p1.off[2]);
+    auto phinu = [&](uint64_t x, int& phi, int& nu) {      // phi: the two slot-index bits
+        uint64_t c2 = 0;
+        if (!gf2_solve(nb, x, c2)) return false;
+        phi = int((c2 >> s1.nk) & 3);
+        const uint64_t c1 = s1.nk >= 64 ? c2 : c2 & ((uint64_t(1) << s1.nk) - 1);
+        nu = parity64(c1 & s1.rhoc);
+        return true;
+    };
+    std::vector<int> phic(static_cast<size_t>(nk)), nuc(static_cast<size_t>(nk));
+    for (int j = 0; j < nk; j++)
+        if (!phinu(s0.cols[size_t(j)], phic[size_t(j)], nuc[size_t(j)])) return false;
+    int phig = 0, nug = 0;
+    if (s0.m == 2 && !phinu(s0.g, phig, nug)) return false;
+    f = p0;
+    memset(f.cols, 0, sizeof f.cols);
+    f.pmask[0] = f.pmask[1] = f.pmask[2] = f.pmask[3] = 0;
+    for (int i = 0; i < nk; i++) {
+        const uint64_t b = basis[size_t(i)];
+        uint64_t col = 0;
+        int pp = 0, pn = 0;
+        for (int j = 0; j < nk; j++)
+            if ((b >> j) & 1) {
+                col ^= s0.cols[size_t(j)];
+                pp ^= phic[size_t(j)];
+                pn ^= nuc[size_t(j)];
+            }
+        f.cols[i] = col;
+        f.pmask[0] |= uint64_t(parity64(b & s0.rhoc)) << i;
+        f.pmask[1] |= uint64_t(pp & 1) << i;
+        f.pmask[2] |= uint64_t(pn) << i;
+        f.pmask[3] |= uint64_t((pp >> 1) & 1) << i;
+    }
+    f.lr = lrf;
+    f.vg = 4;
+    f.route2 = p1.route;
+    f.parmask2 = int(p1.parmask);
+    f.eps = epsv[0];
+    f.eps2 = epsv[1];
+    f.phig = phig;
+    f.nug = nug;
+    set_geometry(f, lrf + 2, e->grid_cap);
     return true;
 }
 
@@ -1071,6 +1151,13 @@ int build_steps(gs_engine* e) {
         if (build_fused(e, k, f)) {                       // next step has two slots: pairing layout
             L.wgram = sweep_kernel_for(e->storage, f.lr, 2, f.ntq, f.route, kWriteGram, f.route2);
             L.wdirect = sweep_kernel_for(e->storage, f.lr, 2, f.ntq, f.route, kWriteDirect, f.route2);
+            if (!L.wgram || !L.wdirect) {
+                L.wgram = L.wdirect = nullptr;
+                continue;
+            }
+        } else if (build_fused4(e, k, f)) {               // next step has four slots: two pairing coordinates
+            L.wgram = sweep_kernel_for(e->storage, f.lr, 4, f.ntq, f.route, kWriteGram, f.route2);
+            L.wdirect = sweep_kernel_for(e->storage, f.lr, 4, f.ntq, f.route, kWriteDirect, f.route2);
             if (!L.wgram || !L.wdirect) {
                 L.wgram = L.wdirect = nullptr;
                 continue;
@@ -1834,6 +1921,15 @@ int gs_set_fusion(gs_engine* eng, int on) {
     if (!eng) return fail(16, "null engine");
     eng->allow_fusion = on != 0;
     return 0;
+}
+
+// Turn the folding of a four-slot next step's norms into the previous write off or on (default on).  Rebuilds the
+// descriptors.
+int gs_set_fused4(gs_engine* eng, int on) {
+    if (!eng) return fail(16, "null engine");
+    eng->allow_fused4 = on != 0;
+    if (int rc = build_steps(eng)) return rc;
+    return upload_descriptors(eng, false);
 }
 
 }  // extern "C"

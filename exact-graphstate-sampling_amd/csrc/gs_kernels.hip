@@ -597,7 +597,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     constexpr int MO = ROUTE == kRoute1 ? 1 : 2;
     constexpr int MO2 = ROUTE2 == kRoute1 ? 1 : 2;
     constexpr int LR = VR == 1 ? 0 : VR == 2 ? 1 : 2;
-    constexpr int LV = LR + (VG == 2 ? 1 : 0);
+    constexpr int LV = LR + (VG == 4 ? 2 : VG == 2 ? 1 : 0);
     constexpr int V = VR * VG;
     constexpr bool WRITES = SWEEP >= kWrite;
     constexpr bool FUSED = SWEEP == kWriteGram || SWEEP == kWriteDirect;
@@ -609,9 +609,12 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     const int tid = threadIdx.x;
     constexpr int lt = LV + 8;                     // log2 cosets per tile
 
-    // parity bits of a coordinate: bit 0 output sign, bit 1 next-step slot index, bit 2 next-step sign
+    // parity bits of a coordinate: bit 0 output sign, bit 1 next-step slot index, bit 2 next-step sign, bit 3 the
+    // next step's second slot-index bit (four-slot next step)
     auto pb = [&](int j) -> int {
-        return int((s.pmask[0] >> j) & 1) | (int((s.pmask[1] >> j) & 1) << 1) | (int((s.pmask[2] >> j) & 1) << 2);
+        int p = int((s.pmask[0] >> j) & 1) | (int((s.pmask[1] >> j) & 1) << 1) | (int((s.pmask[2] >> j) & 1) << 2);
+        if constexpr (VG == 4) p |= int((s.pmask[3] >> j) & 1) << 3;
+        return p;
     };
     uint64_t a_thr = 0;
     int p_thr = 0;
@@ -647,7 +650,8 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
             if ((i >> b) & 1) p ^= pb(b);
         plow[i] = p;
     }
-    const uint64_t gam = VG == 2 ? s.cols[LR] : 0;
+    const uint64_t gam = VG >= 2 ? s.cols[LR] : 0, gam2 = VG == 4 ? s.cols[LR + 1] : 0;
+    auto goff = [&](int ig) -> uint64_t { return ((ig & 1) ? gam : 0) ^ ((ig & 2) ? gam2 : 0); };
 
     double2 w0 = make_double2(0, 0), w1 = w0;
     double2 M[2][2] = {{w0, w0}, {w0, w0}};
@@ -667,7 +671,8 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
     const uint64_t g = s.g;
     const int eps = s.eps, phig = s.phig, nug = s.nug;
     const int sgg = s.sgg, sgk = s.sgk;
-    const uint32_t par2mask = uint32_t(s.par2) << 1;
+    const uint32_t par2mask = VG == 4 ? uint32_t(s.parmask2) : uint32_t(s.par2) << 1;
+    const int eps2 = s.eps2;
 
     double acc[NA];
 #pragma unroll
@@ -687,7 +692,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
         for (int j = 0; j < NTQ; j++)
 #pragma unroll
             for (int ig = 0; ig < VG; ig++)
-                Run<S, VR>::load(mem, (ag ^ off[j] ^ (ig ? gam : 0)) & ~uint64_t(VR - 1), dst[j][ig]);
+                Run<S, VR>::load(mem, (ag ^ off[j] ^ goff(ig)) & ~uint64_t(VR - 1), dst[j][ig]);
     };
     // Tile index i runs 0 .. niter-1, or niter-1 .. 0 when s.rev (gray(i) ^ gray(i-1) = 1 << ctz(i)): the start of a
     // sweep then meets the end of the previous one, whose last lines may still be in the memory-side cache.
@@ -734,7 +739,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
         for (int j = 0; j < NTQ; j++)
 #pragma unroll
             for (int ig = 0; ig < VG; ig++)
-                permute<S, VR>(e[j][ig], unsigned(ag ^ off[j] ^ (ig ? gam : 0)) & (VR - 1));
+                permute<S, VR>(e[j][ig], unsigned(ag ^ off[j] ^ goff(ig)) & (VR - 1));
         S o[2][VG][VR];
 #pragma unroll
         for (int ig = 0; ig < VG; ig++)
@@ -780,7 +785,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
             for (int ig = 0; ig < VG; ig++)
 #pragma unroll
                 for (int t = 0; t < MO; t++) {
-                    const uint64_t a = ag ^ (ig ? gam : 0) ^ (t ? g : 0);
+                    const uint64_t a = ag ^ goff(ig) ^ (t ? g : 0);
                     S so[VR];
 #pragma unroll
                     for (int ir = 0; ir < VR; ir++) so[ir] = o[t][ig][ir];
@@ -788,7 +793,38 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
                     Run<S, VR>::store(mem, a & ~uint64_t(VR - 1), so);
                 }
         }
-        if constexpr (FUSED) {
+        if constexpr (FUSED && VG == 4) {
+            // four-slot next step: the next coset holding output (ir, ig=0, tau d) holds output (ir, ig=m, tau d ^
+            // eps(m)) of the thread's coset m as its slot j0 ^ m, j0 the slot index of the first (both index bits and
+            // the coset's sign are parities of the address)
+#pragma unroll
+            for (int ir = 0; ir < VR; ir++)
+#pragma unroll
+                for (int d = 0; d < MO; d++) {
+                    const int pl = pbase ^ plow[ir];
+                    const int j0 = (((pl >> 1) & 1) | (((pl >> 3) & 1) << 1)) ^ (d ? phig : 0);
+                    const int sg = ((pl >> 2) ^ (d & nug)) & 1;
+                    S a[4];
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        const int tm = d ^ ((m & 1) ? eps : 0) ^ ((m & 2) ? eps2 : 0);
+                        if constexpr (MO == 2) a[m] = tm ? o[1][m][ir] : o[0][m][ir];
+                        else a[m] = o[0][m][ir];
+                    }
+                    // slot q = member q ^ j0, by selects (j0 is per lane)
+                    const S b0 = (j0 & 1) ? a[1] : a[0], b1 = (j0 & 1) ? a[0] : a[1];
+                    const S b2 = (j0 & 1) ? a[3] : a[2], b3 = (j0 & 1) ? a[2] : a[3];
+                    double2 w2[4];
+                    w2[0] = Cx<S>::widen((j0 & 2) ? b2 : b0);
+                    w2[1] = Cx<S>::widen((j0 & 2) ? b3 : b1);
+                    w2[2] = Cx<S>::widen((j0 & 2) ? b0 : b2);
+                    w2[3] = Cx<S>::widen((j0 & 2) ? b1 : b3);
+                    double2 A2[MO2][2];
+                    route_terms<ROUTE2, 4, MO2>(A2, w2, sg, par2mask);
+                    if constexpr (SWEEP == kWriteGram) gram_acc<MO2>(A2, acc + 2);
+                    else direct_acc<MO2>(A2, M, acc + 2);
+                }
+        } else if constexpr (FUSED) {
             // next step's cosets: output (ir, ig=0, tau d) pairs with output (ir, ig=1, tau d ^ eps); the element with
             // slot index 0 is the next coset's base slot, the other its second slot.
 #pragma unroll
@@ -815,7 +851,7 @@ __global__ __launch_bounds__(kThreads) void sweep_kernel(const KStep* __restrict
                 }
         }
     }
-    if constexpr (SWEEP == kWriteGram) gram_assemble<ROUTE2>(gi, s.par2, acc + 2);
+    if constexpr (SWEEP == kWriteGram && VG == 2) gram_assemble<ROUTE2>(gi, s.par2, acc + 2);
     if constexpr (GRAM1) acc[3] = acc[2];
     sweep_epilogue<NA>(acc, part, cur, epi, GRAM1 ? int(kWriteGram) : SWEEP);
 }
@@ -1276,6 +1312,16 @@ void* by_ntq(int ntq, int route, int route2) {
     return nullptr;
 }
 
+// Fused sweeps followed by a four-slot step: mixed routes on both sides (the only ones the planner produces there).
+template <class S, int VR, int NTQ, int SWEEP>
+void* fused4(int route, int route2) {
+    if (route == kRouteMixA)
+        return route2 == kRouteMixA ? kptr<S, VR, 4, NTQ, kRouteMixA, SWEEP, kRouteMixA>()
+                                    : kptr<S, VR, 4, NTQ, kRouteMixA, SWEEP, kRouteMixB>();
+    return route2 == kRouteMixA ? kptr<S, VR, 4, NTQ, kRouteMixB, SWEEP, kRouteMixA>()
+                                : kptr<S, VR, 4, NTQ, kRouteMixB, SWEEP, kRouteMixB>();
+}
+
 template <class S, int SWEEP>
 void* by_layout(int lr, int vg, int ntq, int route, int route2) {
     constexpr bool c64 = sizeof(S) == 8;
@@ -1283,6 +1329,15 @@ void* by_layout(int lr, int vg, int ntq, int route, int route2) {
         return vg == 1 && lr == 1 ? by_ntq<S, 2, 1, SWEEP>(ntq, route, route2) : nullptr;
     } else
     if constexpr (SWEEP == kWriteGram || SWEEP == kWriteDirect) {
+        if (vg == 4) {                           // four-slot next step: two- and four-slot steps, mixed routes only
+            constexpr int VR4 = c64 ? 2 : 1;
+            if (lr != (c64 ? 1 : 0) || (route != kRouteMixA && route != kRouteMixB) ||
+                (route2 != kRouteMixA && route2 != kRouteMixB))
+                return nullptr;
+            if (ntq == 2) return fused4<S, VR4, 2, SWEEP>(route, route2);
+            if (ntq == 4) return fused4<S, VR4, 4, SWEEP>(route, route2);
+            return nullptr;
+        }
         if (vg != 2) return nullptr;
         if constexpr (c64) {
             if (lr == 1) return by_ntq<S, 2, 2, SWEEP>(ntq, route, route2);

@@ -21,7 +21,7 @@ SEED_PLAIN, SEED_SEQ = 0, 1
 ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy", "gs_engine_create",
                "gs_engine_destroy", "gs_prob", "gs_chain", "gs_sample", "gs_hybrid", "gs_prefix", "gs_resume",
                "gs_last_device_ms", "gs_last_error", "gs_profile_enable", "gs_profile_read",
-               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_grid_log2", "gs_set_grid_log2_one_slot", "gs_set_small_steps", "gs_set_walk", "gs_set_xcd_blocks", "gs_set_plain_pairs", "gs_set_fused_run", "gs_set_diag_runs", "gs_step_run", "gs_set_fold_reduce", "gs_race_check", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
+               "gs_step_geometry", "gs_step_columns", "gs_set_fusion", "gs_set_fused4", "gs_set_grid_log2", "gs_set_grid_log2_one_slot", "gs_set_small_steps", "gs_set_walk", "gs_set_xcd_blocks", "gs_set_plain_pairs", "gs_set_fused_run", "gs_set_diag_runs", "gs_step_run", "gs_set_fold_reduce", "gs_race_check", "gs_copy_probe", "gs_batch_create", "gs_batch_destroy", "gs_batch_score",
                "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_set_xcd_items", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms", "gs_tt_plan_load", "gs_tt_plan_info",
                "gs_tt_plan_destroy", "gs_tt_engine_create", "gs_tt_engine_destroy", "gs_tt_prob", "gs_tt_chain",
@@ -73,6 +73,7 @@ def library():
         "gs_profile_read": (I, [P, dp, ctypes.POINTER(I64)]),
         "gs_step_geometry": (I, [P, I, ip]),
         "gs_set_fusion": (I, [P, I]),
+        "gs_set_fused4": (I, [P, I]),
         "gs_step_columns": (I, [P, I, I, ctypes.POINTER(ctypes.c_uint64), I]),
         "gs_set_grid_log2": (I, [P, I]),
         "gs_set_small_steps": (I, [P, I]),
@@ -373,6 +374,10 @@ class Engine:
     def set_fusion(self, on):
         """Fold each sampled step's norm pass into the previous write sweep where possible (default on)."""
         _check(library().gs_set_fusion(self._h, 1 if on else 0))
+
+    def set_fused4(self, on):
+        """Fold a four-slot next step's norms into the previous write too (default on)."""
+        _check(library().gs_set_fused4(self._h, 1 if on else 0))
 
     def set_fused_run(self, lr):
         """log2 run length of the fused layout (complex64: 1 or 2; -1: default)."""

@@ -1,6 +1,12 @@
 """Summaries of rocprofv3 output for the committed profiles.
 
   python rocprof_summary.py stats TRACE_DIR          -> top kernels of a --kernel-trace --stats run (markdown)
+  python rocprof_summary.py launches TRACE_DIR REGEX PERIOD [BYTES ...]
+                                                     -> the dispatches of the kernels whose name matches REGEX, in
+                                                        start order, grouped by position modulo PERIOD (position p:
+                                                        dispatches p, p + PERIOD, ...; e.g. the steps of one pass);
+                                                        with BYTES (one algorithmic byte count per position), the
+                                                        bandwidth and fraction of the 8 TB/s HBM peak per position
   python rocprof_summary.py traffic FETCH_DIR WRITE_DIR MIB
                                                      -> HBM bytes per dispatch per kernel from two --pmc runs
                                                         (FETCH_SIZE, WRITE_SIZE), each scaled by what the counter
@@ -11,6 +17,7 @@ Input directories hold the CSV files rocprofv3 writes with ``-f csv -o run``.
 import collections
 import csv
 import os
+import re
 import sys
 
 
@@ -28,6 +35,32 @@ def stats(trace_dir, top=14):
     for r in rows[:top]:
         out.append(f"| `{_short(r['Name'])[:100]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
                    f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    return "\n".join(out)
+
+
+def launches(trace_dir, regex, period, nbytes=None):
+    rows = [r for r in csv.DictReader(open(os.path.join(trace_dir, "run_kernel_trace.csv")))
+            if re.search(regex, r["Kernel_Name"])]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    pos = collections.defaultdict(list)
+    names = {}
+    for i, r in enumerate(rows):
+        pos[i % period].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        names.setdefault(i % period, _short(r["Kernel_Name"]))
+    head = "| position | kernel | dispatches | avg us | min us | max us |"
+    out = [head + (" TB/s | of 8 TB/s |" if nbytes else ""), "|---" * (6 + (2 if nbytes else 0)) + "|"]
+    for p in range(period):
+        d = pos.get(p, [])
+        if not d:
+            continue
+        avg = sum(d) / len(d)
+        line = f"| {p} | `{names[p][:80]}` | {len(d)} | {avg:.1f} | {min(d):.1f} | {max(d):.1f} |"
+        if nbytes:
+            tbs = nbytes[p] / (avg * 1e-6) / 1e12
+            line += f" {tbs:.2f} | {tbs / 8:.2f} |"
+        out.append(line)
+    if len(rows) % period:
+        out.append(f"\n({len(rows)} dispatches: not a multiple of {period})")
     return "\n".join(out)
 
 
@@ -61,6 +94,9 @@ def traffic(fetch_dir, write_dir, mib):
 if __name__ == "__main__":
     if sys.argv[1] == "stats":
         print(stats(sys.argv[2]))
+    elif sys.argv[1] == "launches":
+        nb = [float(x) for x in sys.argv[5:]] or None
+        print(launches(sys.argv[2], sys.argv[3], int(sys.argv[4]), nb))
     elif sys.argv[1] == "traffic":
         print(traffic(sys.argv[2], sys.argv[3], int(sys.argv[4])))
     else:

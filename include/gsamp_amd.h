@@ -175,6 +175,9 @@ int gs_step_geometry(const gs_engine* eng, int k, int* out);
 int gs_step_columns(const gs_engine* eng, int k, int fused, uint64_t* cols, int cap);
 /* Fold the next step's norm pass into the previous write (on, default) or run it as its own sweep (off). */
 int gs_set_fusion(gs_engine* eng, int on);
+/* Fold the norms of a four-slot next step into the previous write (two pairing coordinates per thread) on or off
+   (default on; off: such a step takes its own norm sweep). */
+int gs_set_fused4(gs_engine* eng, int on);
 /* Cap one sweep at 2^lg blocks (1..14; default 9, plain sweeps of one-slot steps 12, or 14 from 2^30 cosets on); tiles
    beyond the cap loop inside each block.  Sets the one-slot caps to lg as well; lg = 0 restores all three defaults. */
 int gs_set_grid_log2(gs_engine* eng, int lg);

@@ -489,3 +489,43 @@ def test_fold_reduce_bitwise_equal(golden, storage):
     (b0, l0), p0, (c0, q0, a0), (h0, hl0, hc0) = got[False]
     for x, y in ((b1, b0), (l1, l0), (p1, p0), (c1, c0), (q1, q0), (a1, a0), (h1, h0), (hl1, hl0), (hc1, hc0)):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("storage", [gs.C64, gs.C128])
+@pytest.mark.parametrize("plan", ["c4", "ibm"])
+def test_fused_four_slot_next_step(golden, storage, plan):
+    """Writes followed by a four-slot step fold that step's norms in (two pairing coordinates per thread: the thread's
+    four cosets hold whole next-step cosets).  The n = 60 plan (nk up to 18 at such steps) and IBM's width-30 plan:
+    sample, prob, chain and hybrid with the folding on equal the same with those norm passes as their own sweeps --
+    bits identical, complex128 within 1e-12, complex64 within 1e-5 -- and the n = 60 plan's chain-mode log2 Pr equal the
+    reference executor's (golden)."""
+    tol = TOL[storage]
+    if plan == "c4":
+        d = os.path.join(golden, "c4")
+        path = os.path.join(d, "plan.txt")
+        strings = egs.planio.read_strings(os.path.join(d, "strings.txt"), 60)
+    else:
+        d = os.path.join(golden, "ibm")
+        path = os.path.join(d, "nq70_inplace_plan.txt")
+        strings = egs.planio.read_strings(os.path.join(d, "gate_strings.txt"), 70)
+    hy = strings.copy()
+    hy[:, hy.shape[1] // 3:] = -1
+    got = {}
+    print(f"four-slot folded norms, plan={plan} storage={TAG[storage]}", flush=True)
+    with gs.Engine(path, storage) as e:
+        four = [k for k in range(e.ns - 1) if e.geometry(k)["fvg"] == 4]
+        assert len(four) >= 5, four
+        for on in (True, False):
+            e.set_fused4(on)
+            assert any(e.geometry(k)["fvg"] == 4 for k in range(e.ns - 1)) == on
+            got[on] = (e.sample(4, 31), e.prob(strings), e.chain(strings), e.hybrid(hy, 9))
+    (b1, l1), p1, (c1_, q1, a1), (h1, hl1, hc1) = got[True]
+    (b0, l0), p0, (c0, q0, a0), (h0, hl0, hc0) = got[False]
+    assert np.array_equal(b1, b0) and np.array_equal(h1, h0)
+    for x, y in ((l1, l0), (p1, p0), (c1_, c0), (hl1, hl0)):
+        assert close(x, y, 0) <= tol, (x, y)
+    for x, y in ((q1, q0), (hc1, hc0)):
+        assert close(x, y, 0) <= max(tol, 1e-9)
+    if plan == "c4":
+        ref = lines(os.path.join(d, f"chain_{TAG[storage]}.txt"))
+        assert close(c1_, [float(r[0]) for r in ref], 0) <= tol
