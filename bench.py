@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--c5-strings", type=int, default=256)
     ap.add_argument("--c5-c128", type=int, default=4, help="c5 samples/strings re-scored in a complex128 arena")
     ap.add_argument("--no-fusion", action="store_true", help="run every norm pass as its own sweep")
+    ap.add_argument("--fused-run", type=int, default=None, metavar="LR",
+                    help="log2 run length of the fused layout (complex64: 1 or 2; default: engine default)")
     ap.add_argument("--no-fused4", action="store_true",
                     help="norm passes of four-slot steps as their own sweeps (the rest stay folded)")
     ap.add_argument("--fold-reduce", action="store_true",
@@ -118,6 +120,8 @@ def main():
             e.set_fusion(False)
         if args.no_fused4:
             e.set_fused4(False)
+        if args.fused_run is not None:
+            e.set_fused_run(args.fused_run)
         if args.fold_reduce:
             e.set_fold_reduce(True)
         if args.grid_log2:

@@ -1,12 +1,13 @@
 """Summaries of rocprofv3 output for the committed profiles.
 
   python rocprof_summary.py stats TRACE_DIR          -> top kernels of a --kernel-trace --stats run (markdown)
-  python rocprof_summary.py launches TRACE_DIR REGEX PERIOD [BYTES ...]
+  python rocprof_summary.py launches TRACE_DIR REGEX PERIOD [--range A:B] [BYTES ...]
                                                      -> the dispatches of the kernels whose name matches REGEX, in
-                                                        start order, grouped by position modulo PERIOD (position p:
-                                                        dispatches p, p + PERIOD, ...; e.g. the steps of one pass);
-                                                        with BYTES (one algorithmic byte count per position), the
-                                                        bandwidth and fraction of the 8 TB/s HBM peak per position
+                                                        start order (only dispatches A .. B-1 of them with --range),
+                                                        grouped by position modulo PERIOD (position p: dispatches p,
+                                                        p + PERIOD, ...; e.g. the steps of one pass); with BYTES (one
+                                                        algorithmic byte count per position), the bandwidth and
+                                                        fraction of the 8 TB/s HBM peak per position
   python rocprof_summary.py traffic FETCH_DIR WRITE_DIR MIB
                                                      -> HBM bytes per dispatch per kernel from two --pmc runs
                                                         (FETCH_SIZE, WRITE_SIZE), each scaled by what the counter
@@ -38,10 +39,11 @@ def stats(trace_dir, top=14):
     return "\n".join(out)
 
 
-def launches(trace_dir, regex, period, nbytes=None):
+def launches(trace_dir, regex, period, nbytes=None, first=0, stop=None):
     rows = [r for r in csv.DictReader(open(os.path.join(trace_dir, "run_kernel_trace.csv")))
             if re.search(regex, r["Kernel_Name"])]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    rows = rows[first:stop]
     pos = collections.defaultdict(list)
     names = {}
     for i, r in enumerate(rows):
@@ -95,8 +97,11 @@ if __name__ == "__main__":
     if sys.argv[1] == "stats":
         print(stats(sys.argv[2]))
     elif sys.argv[1] == "launches":
-        nb = [float(x) for x in sys.argv[5:]] or None
-        print(launches(sys.argv[2], sys.argv[3], int(sys.argv[4]), nb))
+        rest, first, stop = sys.argv[5:], 0, None
+        if rest[:1] == ["--range"]:
+            a, b = rest[1].split(":")
+            first, stop, rest = int(a), int(b), rest[2:]
+        print(launches(sys.argv[2], sys.argv[3], int(sys.argv[4]), [float(x) for x in rest] or None, first, stop))
     elif sys.argv[1] == "traffic":
         print(traffic(sys.argv[2], sys.argv[3], int(sys.argv[4])))
     else:
