@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--no-fusion", action="store_true", help="run every norm pass as its own sweep")
     ap.add_argument("--fused-run", type=int, default=None, metavar="LR",
                     help="log2 run length of the fused layout (complex64: 1 or 2; default: engine default)")
+    ap.add_argument("--plain-pairs", type=int, default=None, metavar="NTQ",
+                    help="plain sweeps of steps with at most NTQ slots per coset take two runs per thread (0: off)")
     ap.add_argument("--no-fused4", action="store_true",
                     help="norm passes of four-slot steps as their own sweeps (the rest stay folded)")
     ap.add_argument("--fold-reduce", action="store_true",
@@ -122,6 +124,8 @@ def main():
             e.set_fused4(False)
         if args.fused_run is not None:
             e.set_fused_run(args.fused_run)
+        if args.plain_pairs is not None:
+            e.set_plain_pairs(args.plain_pairs)
         if args.fold_reduce:
             e.set_fold_reduce(True)
         if args.grid_log2:
