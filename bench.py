@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--fold-reduce", action="store_true",
                     help="reduce each sweep's block partials in the next sweep instead of in a one-block kernel")
     ap.add_argument("--grid-log2", type=int, default=0, help="cap sweeps at 2^N blocks (default: engine default)")
+    ap.add_argument("--grid-log2-one-slot", type=int, default=0,
+                    help="cap the plain sweeps of one-slot steps and the passes of runs of diagonal steps at 2^N blocks "
+                         "(default: engine default)")
     ap.add_argument("--small-steps", type=int, default=None, metavar="NK",
                     help="runs of steps with at most 2^NK cosets go to one single-block launch (-1: never; default: "
                          "engine default)")
@@ -391,6 +394,8 @@ def workload_c5(gs, egs, args):
     res = {"workload": "c5 width 34: n=80 band 34 seed 5 + 40 gadget ancillas", "N": inst.N, "width": inst.width,
            "R": R, "tb_per_sample_algorithmic": traffic["sample"] / 1e12}
     with gs.Engine(path, gs.C64) as e:
+        if args.grid_log2_one_slot:
+            e.set_grid_log2_one_slot(args.grid_log2_one_slot)
         obs, lpss, lpcs = [], [], []
         ts = tc = 0.0
         for i in range(0, nsamp, 8):                            # chunks, so progress shows every few seconds
