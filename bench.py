@@ -78,8 +78,9 @@ def main():
                     help="plain sweeps of steps with at most NTQ slots per coset take two runs per thread (0: off)")
     ap.add_argument("--no-fused4", action="store_true",
                     help="norm passes of four-slot steps as their own sweeps (the rest stay folded)")
-    ap.add_argument("--fold-reduce", action="store_true",
-                    help="reduce each sweep's block partials in the next sweep instead of in a one-block kernel")
+    ap.add_argument("--fold-reduce", type=int, default=0, metavar="R",
+                    help="reduce a sweep's block partials in the next sweep instead of in a one-block kernel (1: every "
+                         "sweep; R >= 2: next sweeps of steps with at most 2^R live slots; 0: never)")
     ap.add_argument("--grid-log2", type=int, default=0, help="cap sweeps at 2^N blocks (default: engine default)")
     ap.add_argument("--grid-log2-one-slot", type=int, default=0,
                     help="cap the plain sweeps of one-slot steps and the passes of runs of diagonal steps at 2^N blocks "
@@ -130,7 +131,7 @@ def main():
         if args.plain_pairs is not None:
             e.set_plain_pairs(args.plain_pairs)
         if args.fold_reduce:
-            e.set_fold_reduce(True)
+            e.set_fold_reduce(args.fold_reduce)
         if args.grid_log2:
             e.set_grid_log2(args.grid_log2)
         if args.small_steps is not None:

@@ -297,7 +297,9 @@ struct gs_engine {
     int cur_ctl = 0;
     // block partials the last sweep left unreduced (pend_nblk 0: none): the next sweep reduces them itself, or a
     // one-block kernel does before anything else runs
-    bool fold_reduce = false;       // measured slower on the width-26 plan: 4.560 against 4.484 ms per sample
+    int fold_r = -1;                // sweeps of steps of at most 2^fold_r live slots reduce the partials the previous
+                                    // sweep left (-1: none; every sweep measured slower on the width-26 plan: 4.560
+                                    // against 4.484 ms per sample)
     const double* pend_part = nullptr;
     unsigned pend_nblk = 0;
     int pend_sweep = 0;
@@ -380,7 +382,9 @@ int launch_sweep(gs_engine* e, int k, int sweep, const Epi& epi) {
     const int grid = fused ? L.fgrid : L.grid;
     void* mem = e->arena;
     Epi ep = epi;
-    if (e->pend_nblk && !e->fold_reduce)
+    const KStep& hs = (fused ? e->hfused : e->hplain)[size_t(k)];
+    const int live_r = hs.nk + (hs.ntq == 4 ? 2 : hs.ntq == 2 ? 1 : 0);
+    if (e->pend_nblk && (e->fold_r < 0 || live_r > e->fold_r))
         if (int rc = flush_pending(e)) return rc;
     Ctl* ctl = e->ctl;                              // the state this launch starts from
     if (e->pend_nblk) {                             // this sweep reduces the previous one's partials itself
@@ -418,7 +422,7 @@ int launch_sweep(gs_engine* e, int k, int sweep, const Epi& epi) {
         e->pend_epi = ep;
         e->pend_epi.pend_nblk = 0;
         e->cur_part ^= 1;
-        if (!e->fold_reduce) return flush_pending(e);
+        if (e->fold_r < 0) return flush_pending(e);
     }
     return 0;
 }
@@ -1867,12 +1871,13 @@ int gs_step_run(const gs_engine* eng, int k, int* out) {
     return 0;
 }
 
-// Each sweep reduces the block partials the previous sweep left (on), or a one-block kernel reduces them between the
-// two launches (off, default: every block repeating the reduction and chain step costs more than the launch it saves);
-// results are bitwise the same either way.
+// Each sweep reduces the block partials the previous sweep left (on = 1), only the sweeps of steps with at most 2^on
+// live slots do (on >= 2), or a one-block kernel reduces them between the two launches (0, default: every block
+// repeating the reduction and chain step costs more than the launch it saves); results are bitwise the same.
 int gs_set_fold_reduce(gs_engine* eng, int on) {
     if (!eng) return fail(16, "null engine");
-    eng->fold_reduce = on != 0;
+    if (on < 0 || on > 64) return fail(16, "fold setting must be 0..64");
+    eng->fold_r = on == 0 ? -1 : on == 1 ? 64 : on;
     return 0;
 }
 

@@ -367,9 +367,10 @@ class Engine:
         return tot
 
     def set_fold_reduce(self, on):
-        """Each sweep reduces the partial sums the previous sweep left (True) or a one-block kernel does in between
-        (False, default); bitwise the same results."""
-        _check(library().gs_set_fold_reduce(self._h, 1 if on else 0))
+        """Each sweep reduces the partial sums the previous sweep left (True), only the sweeps of steps with at most
+        2^on live slots do (an int >= 2), or a one-block kernel does in between (False, default); bitwise the same
+        results."""
+        _check(library().gs_set_fold_reduce(self._h, int(on) if not isinstance(on, bool) else (1 if on else 0)))
 
     def set_fusion(self, on):
         """Fold each sampled step's norm pass into the previous write sweep where possible (default on)."""

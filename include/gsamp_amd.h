@@ -201,8 +201,8 @@ int gs_set_diag_runs(gs_engine* eng, int max_steps, int min_nk);
 /* The run of diagonal steps holding step k, out[4]: first step, steps, log2 blocks, log2 tiles per block (-1: none). */
 int gs_step_run(const gs_engine* eng, int k, int* out);
 /* A sweep whose blocks leave their partial sums unreduced has them reduced by every block of the next sweep, which
-   then needs no separate reduction launch (on = 1), or by a one-block kernel between the two (0, default); the
-   results are bitwise the same. */
+   then needs no separate reduction launch (on = 1; on >= 2: only when the next sweep's step has at most 2^on live
+   slots), or by a one-block kernel between the two (0, default); the results are bitwise the same. */
 int gs_set_fold_reduce(gs_engine* eng, int on);
 /* Runs of two or more steps with at most 2^nk cosets go to one single-block launch (nk -1: never; default 10). */
 int gs_set_small_steps(gs_engine* eng, int nk);
