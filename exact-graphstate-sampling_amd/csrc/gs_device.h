@@ -98,6 +98,12 @@ struct RStep {
     int nk, j, k0;                      // coordinates, steps, first step
     int kappa;                          // sigma of the coordinate origin (bit i: par of step i)
     int lg_grid, lg_iter, xcd;
+    // pair = 1: the next step is a two-slot step reading this table in pairs (x, x ^ o).  Tile coordinate 0 then has
+    // column o (o ^ 1 when o is odd: element e of a run pairs with element e ^ 1 of the paired run, peps = 1) and both
+    // passes walk it as an explicit pair of tiles; the product pass can sum the next step's Gram matrix, whose slot
+    // index and sign are parities of the address (phim, num: bit c for coordinate c).
+    int pair, peps, route2, par2;
+    uint64_t phim, num;
 };
 
 // Chain state carried on the device between steps (one per engine).
@@ -119,7 +125,8 @@ enum ChainMode { kChain = 0, kChain1 = 1, kRun = 2 };
 
 // What the last block of a sweep does after the sweep (kRoleStored: record the stored table's norm only -- the
 // product pass of a run of diagonal steps, whose decisions and bookkeeping were made before it).
-enum EpiRole { kRoleNorm = 0, kRoleWrite = 1, kRoleStored = 2 };
+// kRoleStoredGram: the same, then the next step's decision from the Gram matrix the product pass summed.
+enum EpiRole { kRoleNorm = 0, kRoleWrite = 1, kRoleStored = 2, kRoleStoredGram = 3 };
 // Write role: what follows step k.
 enum Next { kNextPublish = -2, kNextDecide = -1, kNextFusedGram = kGram, kNextFusedDirect = kDirect, kNextSweep = 3 };
 

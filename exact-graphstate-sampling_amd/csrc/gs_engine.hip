@@ -39,6 +39,8 @@ int small_run_threads();
 void* reduce1_kernel_ptr();
 void* run_hist_kernel_ptr(int storage);
 void* run_apply_kernel_ptr(int storage);
+void* run_apply_gram_kernel_ptr(int storage, int route2);
+void* reduce_stored_gram_kernel_ptr();
 void* run_chunk_kernel_ptr();
 void* run_decide_kernel_ptr();
 }  // namespace gs
@@ -477,7 +479,7 @@ int run_from(const gs_engine* e, int k, int k1) {
 
 // A run of diagonal steps: pattern sums (one pass), their reduction, the run's decisions and bookkeeping (one block),
 // the product pass; its stored norm goes to the chain state.
-int launch_run(gs_engine* e, int ri, const Epi& base) {
+int launch_run(gs_engine* e, int ri, const Epi& base, bool gram) {
     if (int rc = flush_pending(e)) return rc;
     const RStep& rs = e->hrun[size_t(ri)];
     const RStep* rp = e->drun + ri;
@@ -527,10 +529,12 @@ int launch_run(gs_engine* e, int ri, const Epi& base) {
     }
     Epi ea = base;
     ea.k = rs.k0 + rs.j - 1;
-    ea.role = kRoleStored;
+    ea.role = gram ? kRoleStoredGram : kRoleStored;
+    void* apply = gram ? run_apply_gram_kernel_ptr(e->storage, rs.route2) : run_apply_kernel_ptr(e->storage);
+    if (!apply) return fail(15, "no product-pass kernel for the run at step " + std::to_string(rs.k0));
     if (int rc = timed(6, [&]() -> int {
             void* args[] = {&rp, &mem, &facc, &part, &ctl, &ea};
-            HIP_TRY(hipLaunchKernel(run_apply_kernel_ptr(e->storage), dim3(nblk), dim3(kThreads), args, 0, e->stream));
+            HIP_TRY(hipLaunchKernel(apply, dim3(nblk), dim3(kThreads), args, 0, e->stream));
             return 0;
         }))
         return rc;
@@ -538,7 +542,8 @@ int launch_run(gs_engine* e, int ri, const Epi& base) {
         const double* cpart = part;
         int sw = kWrite;
         void* rargs[] = {&cpart, &nblk, &ctl, &ea, &sw};
-        HIP_TRY(hipLaunchKernel(reduce1_kernel_ptr(), dim3(1), dim3(kThreads), rargs, 0, e->stream));
+        HIP_TRY(hipLaunchKernel(gram ? reduce_stored_gram_kernel_ptr() : reduce1_kernel_ptr(), dim3(1), dim3(kThreads),
+                                rargs, 0, e->stream));
     }
     return 0;
 }
@@ -577,9 +582,12 @@ int enqueue_run(gs_engine* e, int mode, int k0, int k1, const int8_t* forced_h, 
         int ke = k;                                       // a run of two or more small steps: one launch
         while (e->small_nk >= 0 && ke < k1 && e->hplain[size_t(ke)].nk <= e->small_nk) ke++;
         if (const int ri = run_from(e, k, k1); ri >= 0) {   // a run of diagonal steps decides its own steps
-            if (int rc = launch_run(e, ri, base)) return rc;
-            decided = false;
-            k += e->hrun[size_t(ri)].j - 1;
+            const RStep& rs = e->hrun[size_t(ri)];
+            const int kn = rs.k0 + rs.j;            // a paired run's product pass can also decide the next step
+            const bool gram = rs.pair && e->allow_fusion && kn < k1 && norm_for(e, mode, kn, fb(kn), fstr) == kGram;
+            if (int rc = launch_run(e, ri, base, gram)) return rc;
+            decided = gram;
+            k += rs.j - 1;
             continue;
         }
         if (ke - k >= 2) {
@@ -950,7 +958,7 @@ std::vector<uint64_t> gf2_kernel(std::vector<uint64_t> rows, int nk) {
 
 // Descriptor of the run of j diagonal steps from step k0 (see RStep), in the coordinates of step k0's table; false
 // when the unit bits 1 .. 256 are not all in the live table or too few sign-neutral tile coordinates exist.
-bool make_run(const gs_engine* e, int k0, int j, RStep& rs) {
+bool make_run(const gs_engine* e, int k0, int j, RStep& rs, bool try_pair) {
     const Plan& P = e->plan;
     const StepInfo& s0 = P.steps[size_t(k0)];
     const std::vector<uint64_t>& C0 = s0.cols;
@@ -985,6 +993,26 @@ bool make_run(const gs_engine* e, int k0, int j, RStep& rs) {
     const int tiles = nk - 9;
     const int cap = nk >= kBigStepLog2 ? e->grid_cap1_big : e->grid_cap1;
     const int lg_grid = std::min(tiles, cap), lg_iter = tiles - lg_grid;
+    // pairing with the next step when it is a two-slot step reading this table in pairs (x, x ^ o): tile coordinate 0
+    // gets column o (o ^ 1 when o is odd), and the next step's slot index and sign become parities per coordinate
+    uint64_t pairv = 0;
+    int peps = 0;
+    bool pair = false;
+    const StepInfo* s1 = k0 + j < P.ns ? &P.steps[size_t(k0 + j)] : nullptr;
+    const KStep* p1 = k0 + j < P.ns ? &e->hplain[size_t(k0 + j)] : nullptr;
+    if (try_pair && lg_iter >= 1 && p1 && p1->ntq == 2 && !p1->growth && p1->off[0] == 0) {
+        uint64_t u = 0;
+        if (gf2_solve(C0, p1->off[1], u)) {
+            if (col_of(u) & 1) {
+                u ^= lowv[0];
+                peps = 1;
+            }
+            if (ech.add(u)) {
+                pairv = u;
+                pair = true;
+            }
+        }
+    }
     // tile coordinates: in the null space of every D_i, with an even physical column
     std::vector<uint64_t> ker = gf2_kernel(D, nk), tilev;
     uint64_t odd = 0;
@@ -996,9 +1024,9 @@ bool make_run(const gs_engine* e, int k0, int j, RStep& rs) {
             }
             v ^= odd;
         }
-        if (int(tilev.size()) < lg_iter && ech.add(v)) tilev.push_back(v);
+        if (int(tilev.size()) < lg_iter - (pair ? 1 : 0) && ech.add(v)) tilev.push_back(v);
     }
-    if (int(tilev.size()) < lg_iter) return false;
+    if (int(tilev.size()) < lg_iter - (pair ? 1 : 0)) return false;
     std::vector<uint64_t> blockv;                     // block coordinates: the plan's where they raise the rank
     for (int q = 0; q < nk && int(blockv.size()) < lg_grid; q++) {
         uint64_t u = uint64_t(1) << q;
@@ -1009,6 +1037,7 @@ bool make_run(const gs_engine* e, int k0, int j, RStep& rs) {
     memset(&rs, 0, sizeof rs);
     std::vector<uint64_t> order(lowv);
     order.insert(order.end(), blockv.begin(), blockv.end());
+    if (pair) order.push_back(pairv);
     order.insert(order.end(), tilev.begin(), tilev.end());
     for (size_t c = 0; c < order.size(); c++) {
         rs.cols[c] = col_of(order[c]);
@@ -1029,6 +1058,29 @@ bool make_run(const gs_engine* e, int k0, int j, RStep& rs) {
     rs.lg_grid = lg_grid;
     rs.lg_iter = lg_iter;
     rs.xcd = e->xcd_blocks;
+    if (pair) {                                       // next step's slot index and sign of each coordinate's column
+        std::vector<uint64_t> nb(s1->cols.begin(), s1->cols.end());
+        nb.push_back(p1->off[1]);
+        bool ok = true;
+        for (size_t c = 0; c < order.size() && ok; c++) {
+            uint64_t c2 = 0;
+            if (!gf2_solve(nb, rs.cols[c], c2)) {
+                ok = false;
+                break;
+            }
+            const uint64_t c1 = s1->nk >= 64 ? c2 : c2 & ((uint64_t(1) << s1->nk) - 1);
+            rs.phim |= ((c2 >> s1->nk) & 1) << c;
+            rs.num |= uint64_t(parity64(c1 & s1->rhoc)) << c;
+        }
+        if (ok) {
+            rs.pair = 1;
+            rs.peps = peps;
+            rs.route2 = p1->route;
+            rs.par2 = int((p1->parmask >> 1) & 1);
+        } else {
+            return false;
+        }
+    }
     // check the pattern bookkeeping against each step's own sign on pseudo-random coordinate vectors
     uint64_t x = 0x9E3779B97F4A7C15ull ^ uint64_t(k0);
     for (int t = 0; t < 64; t++) {
@@ -1079,7 +1131,7 @@ void build_runs(gs_engine* e) {
         int end = k + 1;
         while (end < P.ns && end - k < e->run_max && diag(end) && same_table(k, end)) end++;
         RStep rs;
-        if (end - k >= 2 && make_run(e, k, end - k, rs)) {
+        if (end - k >= 2 && (make_run(e, k, end - k, rs, true) || make_run(e, k, end - k, rs, false))) {
             e->run_at[size_t(k)] = int(e->hrun.size());
             e->hrun.push_back(rs);
             k = end;
@@ -1857,16 +1909,18 @@ int gs_set_diag_runs(gs_engine* eng, int max_steps, int min_nk) {
 }
 
 // The run of diagonal steps that step k belongs to: out[0] its first step, out[1] its length, out[2] log2 blocks of
-// its passes, out[3] log2 tiles per block (all -1 when step k is in no run).
+// its passes, out[3] log2 tiles per block, out[4] 1 when its product pass can sum the next step's Gram matrix (all -1
+// when step k is in no run).
 int gs_step_run(const gs_engine* eng, int k, int* out) {
     if (!eng || k < 0 || k >= eng->plan.ns || !out) return fail(16, "bad step");
-    out[0] = out[1] = out[2] = out[3] = -1;
+    out[0] = out[1] = out[2] = out[3] = out[4] = -1;
     for (const RStep& r : eng->hrun)
         if (k >= r.k0 && k < r.k0 + r.j) {
             out[0] = r.k0;
             out[1] = r.j;
             out[2] = r.lg_grid;
             out[3] = r.lg_iter;
+            out[4] = r.pair;
         }
     return 0;
 }

@@ -462,6 +462,10 @@ __device__ void chain_step(const double (&red)[NQ], Ctl* ctl, const Epi& e, int 
     if constexpr (NQ == 1) {            // one sum: the product pass of a run (role kRoleStored), the stored norm
         c.nstored = red[0];
         return;
+    } else if constexpr (NQ == 5) {     // a product pass that also summed the next step's Gram (kRoleStoredGram)
+        c.nstored = red[0];
+        if (!c.dead) decide(c, e, e.k + 1, kGram, red + 1);
+        return;
     } else if (e.role == kRoleNorm) {
         decide(c, e, e.k, sweep, red);
     } else {
@@ -1069,26 +1073,52 @@ __global__ __launch_bounds__(kThreads) void run_hist_kernel(const RStep* __restr
     run_origin(r, a_tb, sg, active);
     const int jit = 9 + r.lg_grid;
     const int niter = 1 << r.lg_iter;
-    double acc0 = 0, acc1 = 0;
-    uint64_t a_it = 0;
-    for (int it = 0; it < niter; it++) {
-        if (it > 0) a_it ^= r.cols[jit + __builtin_ctz(unsigned(it))];
-        if (!active) continue;
-        S e[2];
-        Run<S, 2>::load(mem, a_tb ^ a_it, e);
-        add_norm(acc0, Cx<S>::widen(e[0]));
-        add_norm(acc1, Cx<S>::widen(e[1]));
-    }
-    __shared__ double sw[2 * kThreads];
-    __shared__ int ss[2 * kThreads];
+    __shared__ double sw[4 * kThreads];
+    __shared__ int ss[4 * kThreads];
     const int t = threadIdx.x;
-    sw[2 * t] = acc0;
-    sw[2 * t + 1] = acc1;
-    ss[2 * t] = sg;
-    ss[2 * t + 1] = sg ^ r.sig[0];
+    int nv = 2;                                     // values per thread
+    if (r.pair) {
+        // tile pairs (a, a ^ gc): the pairing coordinate may change sigma, so the second tile's values go to their
+        // own two patterns
+        const uint64_t gc = r.cols[jit];
+        const int sgp = r.sig[jit];
+        double acc[4] = {0, 0, 0, 0};
+        uint64_t a_it = 0;
+        for (int i = 0; i < (niter >> 1); i++) {
+            if (i > 0) a_it ^= r.cols[jit + 1 + __builtin_ctz(unsigned(i))];
+            if (!active) continue;
+            S e[2], h[2];
+            Run<S, 2>::load(mem, a_tb ^ a_it, e);
+            Run<S, 2>::load(mem, a_tb ^ a_it ^ gc, h);
+            add_norm(acc[0], Cx<S>::widen(e[0]));
+            add_norm(acc[1], Cx<S>::widen(e[1]));
+            add_norm(acc[2], Cx<S>::widen(h[0]));
+            add_norm(acc[3], Cx<S>::widen(h[1]));
+        }
+        for (int q = 0; q < 4; q++) {
+            sw[4 * t + q] = acc[q];
+            ss[4 * t + q] = sg ^ ((q & 1) ? r.sig[0] : 0) ^ ((q & 2) ? sgp : 0);
+        }
+        nv = 4;
+    } else {
+        double acc0 = 0, acc1 = 0;
+        uint64_t a_it = 0;
+        for (int it = 0; it < niter; it++) {
+            if (it > 0) a_it ^= r.cols[jit + __builtin_ctz(unsigned(it))];
+            if (!active) continue;
+            S e[2];
+            Run<S, 2>::load(mem, a_tb ^ a_it, e);
+            add_norm(acc0, Cx<S>::widen(e[0]));
+            add_norm(acc1, Cx<S>::widen(e[1]));
+        }
+        sw[2 * t] = acc0;
+        sw[2 * t + 1] = acc1;
+        ss[2 * t] = sg;
+        ss[2 * t + 1] = sg ^ r.sig[0];
+    }
     __syncthreads();
     double h = 0;
-    for (int q = 0; q < 2 * kThreads; q++)
+    for (int q = 0; q < nv * kThreads; q++)
         if (ss[q] == t) h += sw[q];
     hpart[size_t(blockIdx.x) * kRunBins + t] = h;
 }
@@ -1155,12 +1185,20 @@ __global__ __launch_bounds__(kRunBins) void run_decide_kernel(const RStep* __res
     fac[t] = F;
 }
 
+// Next-step slot index (bit 0) and sign (bit 1) parities of coordinate c of a paired run.
+__device__ __forceinline__ int run_pn(const RStep& r, int c) {
+    return int((r.phim >> c) & 1) | (int((r.num >> c) & 1) << 1);
+}
+
 // Pass 2: every entry times its pattern's factor (FP64, one rounding), in place; the stored table's norm goes to the
-// chain state (role kRoleStored).
-template <class S>
+// chain state (role kRoleStored).  GRAM (paired runs): the pass also sums the next step's Gram matrix from the values
+// it stores -- run element e at address a pairs with element e ^ peps at a ^ gc, the two slots of one next-step coset
+// -- and the chain decides that step (role kRoleStoredGram).
+template <class S, bool GRAM, int ROUTE2>
 __global__ __launch_bounds__(kThreads) void run_apply_kernel(const RStep* __restrict__ rp, S* __restrict__ mem,
                                                             const double2* __restrict__ fac, double* __restrict__ part,
                                                             Ctl* __restrict__ ctl, Epi epi) {
+    constexpr int NA = GRAM ? 5 : 1;
     if (ctl->dead) return;
     const RStep& r = *rp;
     uint64_t a_tb;
@@ -1170,7 +1208,65 @@ __global__ __launch_bounds__(kThreads) void run_apply_kernel(const RStep* __rest
     const double2 f0 = fac[sg], f1 = fac[sg ^ r.sig[0]];
     const int jit = 9 + r.lg_grid;
     const int niter = 1 << r.lg_iter;
-    double acc[1] = {0};
+    double acc[NA];
+#pragma unroll
+    for (int q = 0; q < NA; q++) acc[q] = 0;
+    if (GRAM || r.pair) {
+        const uint64_t gc = r.cols[jit];
+        const int sgp = r.sig[jit];
+        const double2 g0 = fac[sg ^ sgp], g1 = fac[sg ^ sgp ^ r.sig[0]];
+        int pn_tb = 0;                              // next-step parities of the thread's and block's coordinates
+        if constexpr (GRAM) {
+            const int tid = threadIdx.x;
+#pragma unroll
+            for (int b = 0; b < 8; b++)
+                if ((tid >> b) & 1) pn_tb ^= run_pn(r, 1 + b);
+            const unsigned bid = r.xcd && r.lg_grid >= 3 ? (blockIdx.x >> 3) | ((blockIdx.x & 7u) << (r.lg_grid - 3))
+                                                         : blockIdx.x;
+            for (int b = 0; b < r.lg_grid; b++)
+                if ((bid >> b) & 1) pn_tb ^= run_pn(r, 9 + b);
+        }
+        const int pn0 = run_pn(r, 0), peps = r.peps;
+        double gi[kgCount];
+#pragma unroll
+        for (int q = 0; q < kgCount; q++) gi[q] = 0;
+        uint64_t a_it = 0;
+        int pn_it = 0;
+        for (int i = 0; i < (niter >> 1); i++) {
+            if (i > 0) {
+                const int c = jit + 1 + __builtin_ctz(unsigned(i));
+                a_it ^= r.cols[c];
+                if constexpr (GRAM) pn_it ^= run_pn(r, c);
+            }
+            if (!active) continue;
+            const uint64_t a = a_tb ^ a_it, b = a ^ gc;
+            S e[2], h[2];
+            Run<S, 2>::load(mem, a, e);
+            Run<S, 2>::load(mem, b, h);
+            e[0] = Cx<S>::narrow(cmul(Cx<S>::widen(e[0]), f0));
+            e[1] = Cx<S>::narrow(cmul(Cx<S>::widen(e[1]), f1));
+            h[0] = Cx<S>::narrow(cmul(Cx<S>::widen(h[0]), g0));
+            h[1] = Cx<S>::narrow(cmul(Cx<S>::widen(h[1]), g1));
+            Run<S, 2>::store(mem, a, e);
+            Run<S, 2>::store(mem, b, h);
+            add_norm(acc[0], Cx<S>::widen(e[0]));
+            add_norm(acc[0], Cx<S>::widen(e[1]));
+            add_norm(acc[0], Cx<S>::widen(h[0]));
+            add_norm(acc[0], Cx<S>::widen(h[1]));
+            if constexpr (GRAM) {
+#pragma unroll
+                for (int el = 0; el < 2; el++) {
+                    const int pe = pn_tb ^ pn_it ^ (el ? pn0 : 0);      // parities of element el's address
+                    const S vb = (el ^ peps) ? h[1] : h[0];
+                    gram_invariants<ROUTE2>(Cx<S>::widen(e[el]), Cx<S>::widen(vb), uint32_t(pe & 1),
+                                            uint32_t((pe >> 1) & 1), gi);
+                }
+            }
+        }
+        if constexpr (GRAM) gram_assemble<ROUTE2>(gi, r.par2, acc + 1);
+        sweep_epilogue<NA>(acc, part, ctl, epi, kWrite);
+        return;
+    }
     uint64_t a_it = 0;
     for (int it = 0; it < niter; it++) {
         if (it > 0) a_it ^= r.cols[jit + __builtin_ctz(unsigned(it))];
@@ -1184,7 +1280,7 @@ __global__ __launch_bounds__(kThreads) void run_apply_kernel(const RStep* __rest
         add_norm(acc[0], Cx<S>::widen(e[0]));
         add_norm(acc[0], Cx<S>::widen(e[1]));
     }
-    sweep_epilogue<1>(acc, part, ctl, epi, kWrite);
+    sweep_epilogue<NA>(acc, part, ctl, epi, kWrite);
 }
 
 // ---------------------------------------------------------------------------------------------------------------
@@ -1407,9 +1503,24 @@ void* run_hist_kernel_ptr(int storage) {
                         : reinterpret_cast<void*>(&run_hist_kernel<double2>);
 }
 void* run_apply_kernel_ptr(int storage) {
-    return storage == 0 ? reinterpret_cast<void*>(&run_apply_kernel<float2>)
-                        : reinterpret_cast<void*>(&run_apply_kernel<double2>);
+    return storage == 0 ? reinterpret_cast<void*>(&run_apply_kernel<float2, false, kRoute1>)
+                        : reinterpret_cast<void*>(&run_apply_kernel<double2, false, kRoute1>);
 }
+// The product pass of a paired run that also sums the next step's Gram matrix (next route route2).
+template <class S>
+void* run_apply_gram_for(int route2) {
+    switch (route2) {
+        case kRoute1: return reinterpret_cast<void*>(&run_apply_kernel<S, true, kRoute1>);
+        case kRouteSplit: return reinterpret_cast<void*>(&run_apply_kernel<S, true, kRouteSplit>);
+        case kRouteMixA: return reinterpret_cast<void*>(&run_apply_kernel<S, true, kRouteMixA>);
+        case kRouteMixB: return reinterpret_cast<void*>(&run_apply_kernel<S, true, kRouteMixB>);
+    }
+    return nullptr;
+}
+void* run_apply_gram_kernel_ptr(int storage, int route2) {
+    return storage == 0 ? run_apply_gram_for<float2>(route2) : run_apply_gram_for<double2>(route2);
+}
+void* reduce_stored_gram_kernel_ptr() { return reinterpret_cast<void*>(&sweep_reduce_kernel<5>); }
 void* run_chunk_kernel_ptr() { return reinterpret_cast<void*>(&run_chunk_kernel); }
 void* run_decide_kernel_ptr() { return reinterpret_cast<void*>(&run_decide_kernel); }
 int small_run_threads() { return kSmallThreads; }

@@ -333,8 +333,9 @@ class Engine:
         _check(library().gs_set_diag_runs(self._h, int(max_steps), int(min_nk)))
 
     def run_of(self, k):
-        """``(first step, steps, log2 blocks, log2 tiles)`` of the run of diagonal steps holding step k, or None."""
-        out = np.zeros(4, np.int32)
+        """``(first step, steps, log2 blocks, log2 tiles, paired)`` of the run of diagonal steps holding step k, or
+        None; paired: the run's product pass can also sum the next (two-slot) step's Gram matrix."""
+        out = np.zeros(5, np.int32)
         _check(library().gs_step_run(self._h, k, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
         return None if out[0] < 0 else tuple(int(x) for x in out)
 

@@ -198,7 +198,8 @@ int gs_set_plain_pairs(gs_engine* eng, int ntq_max, int far);
    0, the same live table -- of at least 2^min_nk cosets (10..40; default 20): one pass sums |v|^2 per pattern of the
    run's signs, one block makes the run's decisions, one pass multiplies every entry by its pattern's factor. */
 int gs_set_diag_runs(gs_engine* eng, int max_steps, int min_nk);
-/* The run of diagonal steps holding step k, out[4]: first step, steps, log2 blocks, log2 tiles per block (-1: none). */
+/* The run of diagonal steps holding step k, out[5]: first step, steps, log2 blocks, log2 tiles per block, and 1 when
+   its product pass can also sum the next (two-slot) step's Gram matrix (all -1: none). */
 int gs_step_run(const gs_engine* eng, int k, int* out);
 /* A sweep whose blocks leave their partial sums unreduced has them reduced by every block of the next sweep, which
    then needs no separate reduction launch (on = 1; on >= 2: only when the next sweep's step has at most 2^on live

@@ -425,7 +425,8 @@ def test_diag_runs_vs_oracle(gadget_plan, tmp_path, storage, grid_log2):
     2^18 cosets).  Taken as runs (8 + 4 steps: a pattern-sum pass, the decisions in one block, a product pass): hybrid
     samples, prob and chain of random strings, and a prefix dump at K = 30 (inside a run, so those steps go one by one)
     equal the oracle -- bits identical, complex128 within 1e-12, complex64 within 1e-5 -- with 2^9 blocks and one tile
-    each, and with 2^4 blocks looping over 32 sign-neutral tile coordinates."""
+    each, and with 2^4 blocks looping over 32 tiles (then the second run's product pass also sums the Gram matrix of
+    step 38, a two-slot step, and decides it)."""
     import subprocess
     path, inst = gadget_plan
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -452,6 +453,9 @@ def test_diag_runs_vs_oracle(gadget_plan, tmp_path, storage, grid_log2):
         runs = {e.run_of(k) for k in range(ns)} - {None}
         assert {(r[0], r[1]) for r in runs} == {(26, 8), (34, 4)}, runs
         assert all(r[3] == (5 if grid_log2 else 0) for r in runs)
+        # the second run is followed by a two-slot step: with tile coordinates to pair (2^4 blocks), its product pass
+        # also sums that step's Gram matrix for the sampled strings
+        assert {r[0]: r[4] for r in runs} == {26: 0, 34: 1 if grid_log2 else 0}, runs
         ob, lph, condh = e.hybrid(hy, 13)
         lpp = e.prob(rnd)
         lpc, cond, _ = e.chain(rnd)
