@@ -76,6 +76,8 @@ def main():
                     help="log2 run length of the fused layout (complex64: 1 or 2; default: engine default)")
     ap.add_argument("--plain-pairs", type=int, default=None, metavar="NTQ",
                     help="plain sweeps of steps with at most NTQ slots per coset take two runs per thread (0: off)")
+    ap.add_argument("--no-xcd-blocks", action="store_true",
+                    help="width-34 workload: deal blocks round-robin to the XCDs instead of in contiguous eighths")
     ap.add_argument("--no-fused4", action="store_true",
                     help="norm passes of four-slot steps as their own sweeps (the rest stay folded)")
     ap.add_argument("--fold-reduce", type=int, default=0, metavar="R",
@@ -397,6 +399,8 @@ def workload_c5(gs, egs, args):
     with gs.Engine(path, gs.C64) as e:
         if args.grid_log2_one_slot:
             e.set_grid_log2_one_slot(args.grid_log2_one_slot)
+        if args.no_xcd_blocks:
+            e.set_xcd_blocks(False)
         obs, lpss, lpcs = [], [], []
         ts = tc = 0.0
         for i in range(0, nsamp, 8):                            # chunks, so progress shows every few seconds
