@@ -533,3 +533,42 @@ def test_fused_four_slot_next_step(golden, storage, plan):
     if plan == "c4":
         ref = lines(os.path.join(d, f"chain_{TAG[storage]}.txt"))
         assert close(c1_, [float(r[0]) for r in ref], 0) <= tol
+
+
+@pytest.mark.parametrize("storage", [gs.C64, gs.C128])
+def test_tiny_plans_and_empty_batches(tmp_path, storage):
+    """Edge sizes: plans of 1, 2 and 3 vertices (a single step; nk = 0 tables), every string of each scored with prob
+    and chain and 5 samples drawn, against the oracle (bits identical; complex128 1e-12, complex64 1e-5) and summing to
+    probability 1; and empty batches (no strings, no samples) return empty results without touching the device."""
+    import itertools
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    tol = TOL[storage]
+    for n, band in ((1, 1), (2, 1), (3, 2)):
+        inst = egs.instances.Instance(n, band, 0)
+        path = str(tmp_path / f"tiny{n}.txt")
+        inst.write_plan(path)
+        strings = np.array(list(itertools.product([0, 1], repeat=n)), np.int8)
+        egs.planio.write_strings(str(tmp_path / f"s{n}.txt"), strings)
+        cmd = [os.path.join(root, "oracle", "gs_oracle")] + (["--c128"] if storage == gs.C128 else []) + [path]
+        run = lambda *a: [l.split() for l in subprocess.run(cmd + [str(x) for x in a], capture_output=True, text=True,
+                                                            check=True).stdout.splitlines() if l.strip()]
+        ref_s = run("sample", 5, 5)
+        ref_p = [float(r[0]) for r in run("prob", str(tmp_path / f"s{n}.txt"))]
+        ref_c = run("chain", str(tmp_path / f"s{n}.txt"))
+        print(f"tiny plan n={n} storage={TAG[storage]}", flush=True)
+        with gs.Engine(path, storage) as e:
+            assert e.ns == n
+            bits, lp = e.sample(5, 5)
+            lpp = e.prob(strings)
+            lpc, cond, _ = e.chain(strings)
+            eb, el = e.sample(0, 5)
+            assert eb.shape == (0, n) and el.shape == (0,)
+            assert e.prob(strings[:0]).shape == (0,)
+            c0, q0, _ = e.chain(strings[:0])
+            assert c0.shape == (0,) and q0.shape[0] == 0
+        assert [egs.planio.format_bits(b) for b in bits] == [r[0] for r in ref_s]
+        assert close(lp, [float(r[1]) for r in ref_s], 0) <= tol
+        assert close(lpp, ref_p, 0) <= tol
+        assert close(lpc, [float(r[0]) for r in ref_c], 0) <= tol
+        assert abs(np.sum(2.0 ** lpp) - 1.0) <= 10 * tol
