@@ -1,6 +1,6 @@
 """Host planning layer: bit-exact against the reference's committed plans, and correct against brute force.
 
-Gates (byte-identical files from the committed inputs, SURVEY build plan step 2):
+Gates (byte-identical files from the reference's committed inputs):
   * IBM #151 width-30 in-place plan   (cloud_nq76/nq70.qasm + orders/order_w30.json -> cloud_ibmxeb/nq70_inplace_plan.txt)
   * two-table plans                    (cloud_ibmxeb/nq70_w30_plan.txt, cloud/nq70_plan.txt)
   * graph-form adjacency               (results/nq70_graph_adjacency.txt)
