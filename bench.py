@@ -34,6 +34,25 @@ SWEEP_NAMES = ("gram", "direct", "write", "write+gram", "write+direct", "write+g
 ROUTES = ("m1", "split", "mixA", "mixB")
 
 
+def per_step_table(steps, ms, nl, geo, title):
+    """Markdown table of every step's sweeps from a profiled pass: average launch time (HIP events) and bandwidth on the
+    algorithmic bytes of the launch."""
+    out = [title, "", "| step | nk | slots | outputs | sweep | blocks | VG | us | GB | TB/s | of 8 TB/s |",
+           "|---|---|---|---|---|---|---|---|---|---|---|"]
+    for k, s in enumerate(steps):
+        for sw in range(nl.shape[1]):
+            if nl[k, sw] == 0:
+                continue
+            g = geo[k]
+            us = ms[k, sw] / nl[k, sw] * 1e3
+            b = sweep_bytes(s, sw, 8)
+            fused = sw in (3, 4)
+            out.append(f"| {k} | {s['nk']} | {g['ntq']} | {s['m']} | {SWEEP_NAMES[sw]} | "
+                       f"{g['fgrid'] if fused else g['grid']} | {g['fvg'] if fused else g['vg']} | {us:.1f} | "
+                       f"{b / 1e9:.3f} | {b / us / 1e6:.2f} | {b / us / 1e6 / 8:.2f} |")
+    return "\n".join(out)
+
+
 def sweep_bytes(step, sweep, esz):
     """Algorithmic bytes of one sweep launch on a step: norm sweeps read the live slots; write sweeps (also the ones
     that fold in the next step's norms, which read nothing more) read the live slots and write the outputs."""
@@ -78,6 +97,8 @@ def main():
                     help="plain sweeps of steps with at most NTQ slots per coset take two runs per thread (0: off)")
     ap.add_argument("--no-xcd-blocks", action="store_true",
                     help="width-34 workload: deal blocks round-robin to the XCDs instead of in contiguous eighths")
+    ap.add_argument("--per-step", action="store_true",
+                    help="width-26 and IBM workloads: print every step's sweep times and bandwidth (markdown, stderr)")
     ap.add_argument("--no-fused4", action="store_true",
                     help="norm passes of four-slot steps as their own sweeps (the rest stay folded)")
     ap.add_argument("--fold-reduce", type=int, default=0, metavar="R",
@@ -157,6 +178,8 @@ def main():
         e.profile(False)
         assert np.array_equal(bits_p, bits[:KP])
         geo = [e.geometry(k) for k in range(e.ns)]
+        if args.per_step:
+            log(per_step_table(steps, ms, nl, geo, f"per-step sweeps, {KP} samples of the width-26 plan (HIP events)"))
     s_per_sample = wall / K
 
     # dominant kernel: aggregate per template instance <storage, VR, VG, ntq, route, sweep, next route>
@@ -168,8 +191,8 @@ def main():
                 continue
             if sw >= 5:                                                           # a run of diagonal steps
                 key = (2, 1, 1, 0, sw + 1, 0)
-            elif sw >= 3 and gk["fvg"] == 2:
-                key = (1 << gk["flr"], 2, gk["ntq"], gk["route"], sw, gk["route2"])   # fused layout: 2 runs
+            elif sw >= 3 and gk["fvg"] in (2, 4):
+                key = (1 << gk["flr"], gk["fvg"], gk["ntq"], gk["route"], sw, gk["route2"])   # fused layouts
             elif sw >= 3:
                 key = (1 << gk["flr"], 1, gk["ntq"], gk["route"], 5, 0)           # plain layout folding a 1-slot Gram
             else:
@@ -597,6 +620,13 @@ def ibm_w30(gs, egs, K, args):
         t0 = time.perf_counter()
         bits, lp = e.sample(K, 2000000)
         wall = (time.perf_counter() - t0) / K
+        if args.per_step:
+            e.profile(True)
+            e.sample(K, 2000000)
+            ms, nl = e.profile_read()
+            e.profile(False)
+            log(per_step_table(steps, ms, nl, [e.geometry(k) for k in range(e.ns)],
+                               f"per-step sweeps, {K} samples of the IBM width-30 plan (HIP events)"))
     return {"plan": IBM_PLAN, "R": R, "samples": K, "s_per_sample": wall,
             "gb_per_sample_algorithmic": traffic["sample"] / 1e9,
             "hbm_gbps_algorithmic": traffic["sample"] / wall / 1e9,
