@@ -99,6 +99,8 @@ def main():
                     help="width-34 workload: deal blocks round-robin to the XCDs instead of in contiguous eighths")
     ap.add_argument("--per-step", action="store_true",
                     help="width-26 and IBM workloads: print every step's sweep times and bandwidth (markdown, stderr)")
+    ap.add_argument("--tt-grid-log2", type=int, default=0, metavar="LG",
+                    help="two-table engine: cap its gather sweeps at 2^LG blocks (0: engine default)")
     ap.add_argument("--no-fused4", action="store_true",
                     help="norm passes of four-slot steps as their own sweeps (the rest stay folded)")
     ap.add_argument("--fold-reduce", type=int, default=0, metavar="R",
@@ -229,7 +231,7 @@ def main():
     same_bits = int(sum(int(np.array_equal(a, b)) for a, b in zip(bits[:K2], bits2)))
 
     ibm = ibm_w30(gs, egs, args.ibm_steps, args) if args.ibm_steps else None
-    tt2 = ibm_two_table(gs, egs, args.tt_steps) if args.tt_steps else None
+    tt2 = ibm_two_table(gs, egs, args.tt_steps, args.tt_grid_log2) if args.tt_steps else None
 
     legs = {}
     for name in [x for x in args.legs.split(",") if x]:
@@ -636,13 +638,15 @@ def ibm_w30(gs, egs, K, args):
             "first_sample": [egs.planio.format_bits(bits[0]), float(lp[0])]}
 
 
-def ibm_two_table(gs, egs, K):
+def ibm_two_table(gs, egs, K, grid_log2=0):
     """The two-table engine on the reference's committed two-table plans of the IBM circuit: exact samples of the
     width-31 plan (published 13.0 s per sample, README.md:24, 2 x 2^31 complex64 tables) and prob of the two gate
     strings on the width-30 plan (published 6.9 s per string, README.md:138-140)."""
     d = os.path.join(ROOT, "tests", "golden", "ibm")
     out = {}
     with gs.TwoTableEngine(os.path.join(d, "nq70_w31_plan.txt")) as e:
+        if grid_log2:
+            e.set_grid_log2(grid_log2)
         e.sample(1, 7)
         t0 = time.perf_counter()
         e.sample(K, 1000)
@@ -650,6 +654,8 @@ def ibm_two_table(gs, egs, K):
     out["w31_sample"] = {"s_per_sample": ts, "samples": K, "reference_published_s": 13.0,
                          "speedup_vs_published": 13.0 / ts}
     with gs.TwoTableEngine(os.path.join(d, "nq70_w30_plan.txt")) as e:
+        if grid_log2:
+            e.set_grid_log2(grid_log2)
         strings = egs.planio.read_strings(os.path.join(d, "gate_strings.txt"), e.ns)
         e.prob(strings[:1])
         t0 = time.perf_counter()

@@ -14,6 +14,8 @@ constexpr int kDefaultGridLog2One = 12; // plain sweeps of one-slot steps: one l
                                         // (width 34, 1-slot norm passes: 691 -> 343 ms per sample at 2^12 blocks)
 constexpr int kBigStepLog2 = 30;        // one-slot steps of at least 2^30 cosets take more blocks still:
 constexpr int kDefaultGridLog2OneBig = 14;  // width 34: 1.231 -> 1.196 s per sample; width 30 prefers 2^12
+constexpr int kDefaultTTGridLog2 = 14;  // gather sweeps of the two-table engine (latency-bound gathers: the
+                                         // width-31 IBM sample 0.276 s at 2^9 blocks, 0.121 s at 2^14)
 constexpr int kFenceBlocks = 64;    // sweeps of at most this many blocks reduce in their last block
 constexpr int kPartStride = 8;      // doubles per block partial (write norms + fused next-step norms)
 constexpr size_t kPartDoubles = size_t(kPartStride) << kMaxGridLog2;

@@ -1290,7 +1290,7 @@ __global__ __launch_bounds__(kThreads) void run_apply_kernel(const RStep* __rest
 // Write pass (kWrite): w0 S0 + w1 S1 stored at nw[cp] (contiguous), with w from the chain's decision or, for the
 // amplitude engine, from wext[4k..4k+3] (the job's per-vertex weights).  Threads own consecutive outputs: output
 // bits 0..7 are the thread index, then the block index, then the Gray-ordered tile index.
-template <class S, int SWEEP>
+template <class S, int SWEEP, int NKS>
 __global__ __launch_bounds__(kThreads) void gather_kernel(const TStep* __restrict__ tp, const S* __restrict__ old,
                                                           S* __restrict__ nw, double* __restrict__ part,
                                                           Ctl* __restrict__ ctl, Epi epi,
@@ -1320,30 +1320,21 @@ __global__ __launch_bounds__(kThreads) void gather_kernel(const TStep* __restric
         w0 = make_double2(w[0], w[1]);
         w1 = make_double2(w[2], w[3]);
     }
-    const int nks = s.nks;
-    uint64_t ks[4];
+    uint64_t ks[NKS];
 #pragma unroll
-    for (int t = 0; t < 4; t++) ks[t] = s.kspan[t];
+    for (int t = 0; t < NKS; t++) ks[t] = s.kspan[t];
     const uint64_t lowmask = s.lowmask, xbit = s.xbit, rho = s.rho;
     double acc[2] = {0, 0};
-    uint64_t a_it = 0, c_it = 0;
-    for (int it = 0; it < niter; it++) {
-        if (it) {
-            const int b = __builtin_ctz(unsigned(it));
-            a_it ^= s.dep[jit + b];
-            c_it ^= uint64_t(1) << (jit + b);
-        }
-        if (!active) continue;
-        const uint64_t u0 = a_thr ^ a_blk ^ a_it;
+    // One output from its NKS gathered inputs (u0: its address before the kernel span): S0, S1, then the branch norms
+    // (norm pass) or the write at cp.
+    auto output = [&](const S (&v)[NKS], uint64_t u0, uint64_t cp) {
         double2 S0 = make_double2(0, 0), S1 = S0;
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            if (t >= nks) break;
+        for (int t = 0; t < NKS; t++) {
             const uint64_t u = u0 ^ ks[t];
-            const uint64_t c = u & lowmask;
-            const double2 v = Cx<S>::widen(old[c]);
-            if (u & xbit) S1 = cadd(S1, negate_if(v, __popcll(c & rho) & 1));
-            else S0 = cadd(S0, v);
+            const double2 x = Cx<S>::widen(v[t]);
+            if (u & xbit) S1 = cadd(S1, negate_if(x, __popcll(u & lowmask & rho) & 1));
+            else S0 = cadd(S0, x);
         }
         if constexpr (SWEEP == kDirect) {
             add_norm(acc[0], cfma(M[0][0], S0, cmul(M[0][1], S1)));
@@ -1351,10 +1342,52 @@ __global__ __launch_bounds__(kThreads) void gather_kernel(const TStep* __restric
         } else {
             const double2 o = cfma(w0, S0, cmul(w1, S1));
             const S so = Cx<S>::narrow(o);
-            const uint64_t cp = c_it | (uint64_t(blockIdx.x) << 8) | uint64_t(tid);
             Run<S, 1>::store(nw, cp, {so});
             add_norm(acc[0], o);
             add_norm(acc[1], Cx<S>::widen(so));
+        }
+    };
+    const uint64_t ab = a_thr ^ a_blk;
+    const uint64_t cb = (uint64_t(blockIdx.x) << 8) | uint64_t(tid);
+    constexpr int U = 4;        // tiles per iteration (every gather of the U tiles in flight at once)
+    if (niter >= U) {
+        // tile index (gray(g) << 2) | j: the two low tile bits inside the group, a Gray walk over the groups
+        const uint64_t d0 = s.dep[jit], d1 = s.dep[jit + 1];
+        uint64_t a_g = 0, c_g = 0;
+        for (int g = 0; g < niter / U; g++) {
+            if (g) {
+                const int b = 2 + __builtin_ctz(unsigned(g));
+                a_g ^= s.dep[jit + b];
+                c_g ^= uint64_t(1) << (jit + b);
+            }
+            if (!active) continue;
+            S v[U][NKS];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                const uint64_t u0 = ab ^ a_g ^ ((j & 1) ? d0 : 0) ^ ((j & 2) ? d1 : 0);
+#pragma unroll
+                for (int t = 0; t < NKS; t++) v[j][t] = old[(u0 ^ ks[t]) & lowmask];
+            }
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                const uint64_t u0 = ab ^ a_g ^ ((j & 1) ? d0 : 0) ^ ((j & 2) ? d1 : 0);
+                output(v[j], u0, c_g | (uint64_t(j) << jit) | cb);
+            }
+        }
+    } else {
+        uint64_t a_it = 0, c_it = 0;
+        for (int it = 0; it < niter; it++) {
+            if (it) {
+                const int b = __builtin_ctz(unsigned(it));
+                a_it ^= s.dep[jit + b];
+                c_it ^= uint64_t(1) << (jit + b);
+            }
+            if (!active) continue;
+            const uint64_t u0 = ab ^ a_it;
+            S v[NKS];
+#pragma unroll
+            for (int t = 0; t < NKS; t++) v[t] = old[(u0 ^ ks[t]) & lowmask];
+            output(v, u0, c_it | cb);
         }
     }
     sweep_epilogue<2>(acc, part, ctl, epi, SWEEP);
@@ -1525,11 +1558,22 @@ void* run_chunk_kernel_ptr() { return reinterpret_cast<void*>(&run_chunk_kernel)
 void* run_decide_kernel_ptr() { return reinterpret_cast<void*>(&run_decide_kernel); }
 int small_run_threads() { return kSmallThreads; }
 // Two-table kernels: sweep kDirect (norm pass) or kWrite; storage 0 = complex64, 1 = complex128.
-void* gather_kernel_ptr(int storage, int sweep) {
-    if (storage == 0) return sweep == kDirect ? reinterpret_cast<void*>(&gather_kernel<float2, kDirect>)
-                                              : reinterpret_cast<void*>(&gather_kernel<float2, kWrite>);
-    return sweep == kDirect ? reinterpret_cast<void*>(&gather_kernel<double2, kDirect>)
-                            : reinterpret_cast<void*>(&gather_kernel<double2, kWrite>);
+template <class S, int NKS>
+void* gather_for(int sweep) {
+    return sweep == kDirect ? reinterpret_cast<void*>(&gather_kernel<S, kDirect, NKS>)
+                            : reinterpret_cast<void*>(&gather_kernel<S, kWrite, NKS>);
+}
+template <class S>
+void* gather_by_nks(int sweep, int nks) {
+    switch (nks) {
+        case 1: return gather_for<S, 1>(sweep);
+        case 2: return gather_for<S, 2>(sweep);
+        case 4: return gather_for<S, 4>(sweep);
+    }
+    return nullptr;
+}
+void* gather_kernel_ptr(int storage, int sweep, int nks) {
+    return storage == 0 ? gather_by_nks<float2>(sweep, nks) : gather_by_nks<double2>(sweep, nks);
 }
 void* table_one_kernel_ptr(int storage) {
     return storage == 0 ? reinterpret_cast<void*>(&table_one_kernel<float2>)

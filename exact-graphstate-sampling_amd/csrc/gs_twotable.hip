@@ -22,7 +22,7 @@
 using namespace gs;
 
 namespace gs {
-void* gather_kernel_ptr(int storage, int sweep);
+void* gather_kernel_ptr(int storage, int sweep, int nks);
 void* table_one_kernel_ptr(int storage);
 void* decide_kernel_ptr();
 void* publish_kernel_ptr();
@@ -95,6 +95,7 @@ struct gs_tt_engine {
     double* h_w = nullptr;
     unsigned char* h_amp = nullptr;
     int cur = 0;                         // table holding the current step's input after the last enqueue
+    int grid_cap = kDefaultTTGridLog2;   // log2 of the most blocks a gather sweep launches
 };
 
 namespace {
@@ -173,7 +174,9 @@ int launch_gather(gs_tt_engine* e, int k, int sweep, int in, const Epi& epi, con
     const double* w = wext;
     void* args[] = {&sp, &old, &nw, &part, &ctl, &ep, &w};
     const int grid = 1 << e->hsteps[size_t(k)].lg_grid;
-    TT_TRY(hipLaunchKernel(gather_kernel_ptr(e->storage, sweep), dim3(grid), dim3(kThreads), args, 0, e->stream));
+    void* fn = gather_kernel_ptr(e->storage, sweep, e->hsteps[size_t(k)].nks);
+    if (!fn) return ttfail(15, "no gather kernel for step " + std::to_string(k));
+    TT_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kThreads), args, 0, e->stream));
     if (grid > kFenceBlocks) {
         const double* cpart = part;
         unsigned nblk = unsigned(grid);
@@ -303,6 +306,15 @@ int run_forced(gs_tt_engine* e, int mode, const int8_t* bits, int64_t nstr, doub
     return 0;
 }
 
+// Blocks and tiles per block of every step's gather sweep: 2^rn outputs, one per thread and tile.
+void tt_geometry(gs_tt_engine* e) {
+    for (auto& t : e->hsteps) {
+        const int tiles = std::max(0, t.rn - 8);
+        t.lg_grid = std::min(tiles, e->grid_cap);
+        t.lg_iter = tiles - t.lg_grid;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -368,15 +380,13 @@ int gs_tt_engine_create(const gs_tt_plan* plan, int storage, int device, gs_tt_e
         t.xbit = uint64_t(1) << s.r;
         t.rho = s.rho;
         t.rn = s.rn;
-        const int tiles = std::max(0, s.rn - 8);
-        t.lg_grid = std::min(tiles, kDefaultGridLog2);
-        t.lg_iter = tiles - t.lg_grid;
         for (int b = 0; b < 2; b++)
             for (int x = 0; x < 2; x++) {
                 hplain[size_t(k)].mre[b][x] = s.M[b][x].real();
                 hplain[size_t(k)].mim[b][x] = s.M[b][x].imag();
             }
     }
+    tt_geometry(e);
     const size_t tbytes = (size_t(1) << e->plan.maxr) * e->esz;
     for (int i = 0; i < 2; i++)
         if (hipMalloc(&e->tab[i], tbytes) != hipSuccess) {
@@ -409,6 +419,19 @@ int gs_tt_engine_create(const gs_tt_plan* plan, int storage, int device, gs_tt_e
 }
 
 void gs_tt_engine_destroy(gs_tt_engine* eng) { free_tt(eng); }
+
+int gs_tt_set_grid_log2(gs_tt_engine* eng, int lg) {
+    if (!eng) return ttfail(16, "null engine");
+    if (lg < 0 || lg > kMaxGridLog2) return ttfail(16, "grid log2 must be in 0.." + std::to_string(kMaxGridLog2));
+    if (hipSetDevice(eng->device) != hipSuccess || hipStreamSynchronize(eng->stream) != hipSuccess)
+        return ttfail(17, "device synchronisation failed");
+    eng->grid_cap = lg ? lg : kDefaultTTGridLog2;
+    tt_geometry(eng);
+    if (hipMemcpy(eng->dsteps, eng->hsteps.data(), sizeof(TStep) * eng->hsteps.size(), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return ttfail(17, "step upload failed");
+    return 0;
+}
 
 int gs_tt_prob(gs_tt_engine* eng, const int8_t* bits, int64_t nstr, double* log2p) {
     if (!eng || !eng->plan.with_m || nstr < 0 || (nstr > 0 && (!bits || !log2p))) return ttfail(16, "bad argument");

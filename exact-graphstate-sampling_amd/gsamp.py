@@ -25,7 +25,7 @@ ABI_SYMBOLS = ("gs_plan_load", "gs_plan_parse", "gs_plan_info", "gs_plan_destroy
                "gs_batch_use_graph", "gs_batch_set_tile", "gs_batch_set_unroll", "gs_batch_set_xcd_items", "gs_batch_last_ms", "gs_batch_last_error",
                "gs_batch_set_matrices", "gs_batch_grad", "gs_batch_grad_ms", "gs_tt_plan_load", "gs_tt_plan_info",
                "gs_tt_plan_destroy", "gs_tt_engine_create", "gs_tt_engine_destroy", "gs_tt_prob", "gs_tt_chain",
-               "gs_tt_sample", "gs_tt_hybrid", "gs_tt_prefix", "gs_tt_resume", "gs_tt_dp", "gs_tt_last_error",
+               "gs_tt_sample", "gs_tt_hybrid", "gs_tt_prefix", "gs_tt_resume", "gs_tt_dp", "gs_tt_last_error", "gs_tt_set_grid_log2",
                "gs_prob_noalloc")
 
 _lib = None
@@ -100,6 +100,7 @@ def library():
         "gs_batch_set_unroll": (I, [P, I]),
         "gs_tt_plan_load": (I, [ctypes.c_char_p, I, pp]),
         "gs_tt_plan_info": (I, [P, ip, ip, ip]),
+        "gs_tt_set_grid_log2": (I, [P, I]),
         "gs_tt_plan_destroy": (None, [P]),
         "gs_tt_engine_create": (I, [P, I, I, pp]),
         "gs_tt_engine_destroy": (None, [P]),
@@ -554,6 +555,10 @@ class TwoTableEngine:
     def _check(self, rc):
         if rc:
             raise GsError(rc, library().gs_tt_last_error().decode())
+
+    def set_grid_log2(self, lg):
+        """Cap every gather sweep at 2^lg blocks (1..14; 0: the default)."""
+        self._check(library().gs_tt_set_grid_log2(self._h, int(lg)))
 
     def close(self):
         if _lib is not None:

@@ -146,6 +146,9 @@ int gs_tt_resume(gs_tt_engine* eng, const int8_t* bits, int K, int nslices, cons
  * w [total slices][m][4] (w0re w0im w1re w1im per vertex), out [njobs][2]. */
 int gs_tt_dp(gs_tt_engine* eng, int64_t njobs, const int* nslices, const double* pre, const double* w, double* out);
 const char* gs_tt_last_error(void);
+/* Cap every gather sweep at 2^lg blocks (1..14; 0 restores the default 14); tiles beyond the cap loop inside
+   each block.  No counterpart in the reference (a tuning knob of this engine). */
+int gs_tt_set_grid_log2(gs_tt_engine* eng, int lg);
 
 /* prob mode without any device allocation.  No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

@@ -83,6 +83,30 @@ def test_ibm_chain_known_answers(tdir, golden):
     assert np.max(np.abs(lpr - want)) < 1e-5, (lpr, want)
 
 
+@pytest.mark.parametrize("storage,tol", [(gs.C64, 1e-5), (gs.C128, 1e-12)])
+def test_grid_caps_agree(golden, storage, tol):
+    """The gather sweeps compute every output the same way whatever the launch shape; the block count changes only the
+    order of the FP64 norm sums, hence the last bits of each step's scale and, in complex64, the rounding of the
+    scaled table.  On IBM's width-30 plan, prob and chain of the gate strings and two samples at 2^9, 2^12 and the
+    default 2^14 blocks: identical bits, log2 Pr within 1e-5 (complex64) / 1e-12 (complex128); a cap beyond 2^14 is
+    rejected."""
+    print(f"two-table IBM w30: grid caps 2^9, 2^12, default, storage {storage}", flush=True)
+    got = {}
+    with gs.TwoTableEngine(os.path.join(golden, "ibm", "nq70_w30_plan.txt"), storage=storage) as e:
+        gate = egs.planio.read_strings(os.path.join(golden, "ibm", "gate_strings.txt"), e.ns)
+        for lg in (9, 12, 0):
+            e.set_grid_log2(lg)
+            got[lg] = (e.prob(gate), e.chain(gate)[0], e.sample(2, 77))
+        with pytest.raises(gs.GsError):
+            e.set_grid_log2(15)
+    p0, c0, (b0, l0) = got[0]
+    for lg in (9, 12):
+        p, c, (b, l) = got[lg]
+        assert np.array_equal(b, b0)
+        for x, y in ((p, p0), (c, c0), (l, l0)):
+            assert np.max(np.abs(np.asarray(x) - np.asarray(y))) < tol, (lg, x, y)
+
+
 def test_cli_front_ends(tdir, golden, tmp_path):
     """gsamp_amd and dp_amd print the reference's lines."""
     bindir = os.path.join(ROOT, "exact-graphstate-sampling_amd", "bin")
