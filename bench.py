@@ -70,10 +70,10 @@ def sweep_bytes(step, sweep, esz):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "c5", "ibm"),
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "c5", "ibm", "tt"),
                     help="c2: exact sampling at width 30 (the headline line); c3 batched scoring; c4 gradients; "
-                         "c5 width-34 sampling; ibm: samples of the reference's width-30 IBM plan (each prints its "
-                         "own JSON line)")
+                         "c5 width-34 sampling; ibm: samples of the reference's width-30 IBM plan; tt: the two-table "
+                         "engine on the reference's IBM two-table plans (each prints its own JSON line)")
     ap.add_argument("--steps", type=int, default=256)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--profile-steps", type=int, default=32, help="samples in the profiled (per-kernel timing) pass")
@@ -132,7 +132,9 @@ def main():
               f"(read + write)", file=sys.stderr, flush=True)
     if args.workload != "c2":
         out = {"c3": workload_c3, "c4": workload_c4, "c5": workload_c5,
-               "ibm": lambda gs, egs, args: ibm_w30(gs, egs, args.ibm_steps, args)}[args.workload](gs, egs, args)
+               "ibm": lambda gs, egs, args: ibm_w30(gs, egs, args.ibm_steps, args),
+               "tt": lambda gs, egs, args: ibm_two_table(gs, egs, args.tt_steps, args.tt_grid_log2)}[args.workload](
+            gs, egs, args)
         print(json.dumps(out))
         return
     inst = egs.instances.config_instance("c2")

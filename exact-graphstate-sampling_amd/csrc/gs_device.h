@@ -80,6 +80,11 @@ struct TStep {
     int nks;               // 2^nker
     int lg_grid;           // log2 blocks
     int lg_iter;           // log2 tiles per block
+    // Partner outputs: cp and cp ^ pdelta gather the same entries (input address u ^ pD = u ^ xbit ^ kspan[ptd],
+    // equal to u ^ kspan[ptd] below the x bit), so one thread takes both and loads them once; ph = the top bit of
+    // pdelta, a tile bit, or -1 (no pairing).
+    uint64_t pdelta, pD;
+    int ph, ptd;
 };
 
 // A run of consecutive diagonal steps: one-slot, one-output steps with slot offset 0 over the same live table.  Step i

@@ -1351,27 +1351,53 @@ __global__ __launch_bounds__(kThreads) void gather_kernel(const TStep* __restric
     const uint64_t cb = (uint64_t(blockIdx.x) << 8) | uint64_t(tid);
     constexpr int U = 4;        // tiles per iteration (every gather of the U tiles in flight at once)
     if (niter >= U) {
-        // tile index (gray(g) << 2) | j: the two low tile bits inside the group, a Gray walk over the groups
-        const uint64_t d0 = s.dep[jit], d1 = s.dep[jit + 1];
+        // Tiles of a group: j & 1 selects output bit jit, j >> 1 the second direction -- the partner direction
+        // pdelta when the step has one (tiles j = 2, 3 then reuse the loads of tiles 0, 1, permuted by ptd: each
+        // table entry is read once instead of twice), else output bit jit + 1.  The other tile bits (jit + 1 ..
+        // rn - 1 without h, the second direction's top bit) take a Gray walk over the groups.
+        const bool share = s.ph >= 0;
+        const int h = share ? s.ph : jit + 1;
+        const uint64_t d0 = s.dep[jit];
+        const uint64_t D = share ? s.pD : s.dep[jit + 1];
+        const uint64_t dc = share ? s.pdelta : uint64_t(1) << (jit + 1);
+        const int td = s.ptd;
         uint64_t a_g = 0, c_g = 0;
         for (int g = 0; g < niter / U; g++) {
             if (g) {
-                const int b = 2 + __builtin_ctz(unsigned(g));
-                a_g ^= s.dep[jit + b];
-                c_g ^= uint64_t(1) << (jit + b);
+                int b = jit + 1 + __builtin_ctz(unsigned(g));
+                if (b >= h) b++;
+                a_g ^= s.dep[b];
+                c_g ^= uint64_t(1) << b;
             }
             if (!active) continue;
             S v[U][NKS];
+            if (share) {
 #pragma unroll
-            for (int j = 0; j < U; j++) {
-                const uint64_t u0 = ab ^ a_g ^ ((j & 1) ? d0 : 0) ^ ((j & 2) ? d1 : 0);
+                for (int j = 0; j < 2; j++) {
+                    const uint64_t u0 = ab ^ a_g ^ (j ? d0 : 0);
 #pragma unroll
-                for (int t = 0; t < NKS; t++) v[j][t] = old[(u0 ^ ks[t]) & lowmask];
+                    for (int t = 0; t < NKS; t++) v[j][t] = old[(u0 ^ ks[t]) & lowmask];
+#pragma unroll
+                    for (int t = 0; t < NKS; t++) {           // v[j][t ^ td] by selects (no indexed registers)
+                        S x = v[j][0];
+#pragma unroll
+                        for (int q = 1; q < NKS; q++)
+                            if ((t ^ td) == q) x = v[j][q];
+                        v[j + 2][t] = x;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < U; j++) {
+                    const uint64_t u0 = ab ^ a_g ^ ((j & 1) ? d0 : 0) ^ ((j & 2) ? D : 0);
+#pragma unroll
+                    for (int t = 0; t < NKS; t++) v[j][t] = old[(u0 ^ ks[t]) & lowmask];
+                }
             }
 #pragma unroll
             for (int j = 0; j < U; j++) {
-                const uint64_t u0 = ab ^ a_g ^ ((j & 1) ? d0 : 0) ^ ((j & 2) ? d1 : 0);
-                output(v[j], u0, c_g | (uint64_t(j) << jit) | cb);
+                const uint64_t u0 = ab ^ a_g ^ ((j & 1) ? d0 : 0) ^ ((j & 2) ? D : 0);
+                output(v[j], u0, (c_g | (uint64_t(j & 1) << jit) | cb) ^ ((j & 2) ? dc : 0));
             }
         }
     } else {

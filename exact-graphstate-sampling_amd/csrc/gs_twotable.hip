@@ -306,12 +306,39 @@ int run_forced(gs_tt_engine* e, int mode, const int8_t* bits, int64_t nstr, doub
     return 0;
 }
 
-// Blocks and tiles per block of every step's gather sweep: 2^rn outputs, one per thread and tile.
+// Blocks and tiles per block of every step's gather sweep (2^rn outputs, one per thread and tile), and its partner
+// direction: an output-space vector pdelta whose image pD = pdep(pdelta) is xbit ^ kspan[q] for some q.  Output
+// u0 ^ pD then gathers the entries (u0 ^ kspan[t ^ q]) & lowmask, the same as output u0: the x bit is not an address
+// bit of the old table.  Used when the top bit of pdelta is a tile bit above the first (the kernel's groups of 4
+// tiles pair j with j + 2 along pdelta).
 void tt_geometry(gs_tt_engine* e) {
     for (auto& t : e->hsteps) {
         const int tiles = std::max(0, t.rn - 8);
         t.lg_grid = std::min(tiles, e->grid_cap);
         t.lg_iter = tiles - t.lg_grid;
+        t.ph = -1;
+        t.pdelta = t.pD = 0;
+        t.ptd = 0;
+        if (t.lg_iter < 2) continue;
+        const int jit = 8 + t.lg_grid;
+        for (int q = 0; q < t.nks; q++) {
+            const uint64_t D = t.xbit ^ t.kspan[q];
+            uint64_t delta = 0, rest = D;
+            for (int j = 0; j < t.rn; j++)
+                if (D & t.dep[j]) {
+                    delta |= uint64_t(1) << j;
+                    rest &= ~t.dep[j];
+                }
+            if (rest || !delta) continue;             // D outside the outputs' image
+            const int h = 63 - __builtin_clzll(delta);
+            if (h >= jit + 1) {
+                t.ph = h;
+                t.pdelta = delta;
+                t.pD = D;
+                t.ptd = q;
+            }
+            break;
+        }
     }
 }
 
