@@ -12,6 +12,13 @@
                                                      -> HBM bytes per dispatch per kernel from two --pmc runs
                                                         (FETCH_SIZE, WRITE_SIZE), each scaled by what the counter
                                                         reads for the copy probe of MIB MiB (bench.py --calibrate-copy)
+  python rocprof_summary.py gather TRACE_DIR FETCH_DIR WRITE_DIR PLAN MIB SAMPLES
+                                                     -> the two-table engine's exact samples of PLAN (a gsamp plan:
+                                                        one norm pass and one write pass per step): samples 1 ..
+                                                        SAMPLES-1 of the trace (sample 0 is the warm-up), per step and
+                                                        pass the kernel time, the algorithmic bytes (each old-table
+                                                        entry a pass gathers read once, the writes) and the HBM bytes
+                                                        of the counter runs, calibrated as in traffic
 
 Input directories hold the CSV files rocprofv3 writes with ``-f csv -o run``.
 """
@@ -93,6 +100,54 @@ def traffic(fetch_dir, write_dir, mib):
     return "\n".join(out)
 
 
+def _gather_rows(path, key):
+    rows = [r for r in csv.DictReader(open(path)) if "gather_kernel" in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    return rows
+
+
+def gather(trace_dir, fetch_dir, write_dir, plan, mib, samples):
+    toks = open(plan).read().split()
+    ns, i, pos = int(toks[1]), 3, []
+    for k in range(ns):
+        r, rn, nk = int(toks[i + 1]), int(toks[i + 2]), int(toks[i + 5])
+        i += 6 + nk + 8
+        rd = min(1 << r, 1 << (rn + nk)) * 8                  # c64 entries gathered, each read once
+        pos += [(k, "norm", r, rn, nk, rd), (k, "write", r, rn, nk, rd + (1 << rn) * 8)]
+    P = len(pos)
+    tr = _gather_rows(os.path.join(trace_dir, "run_kernel_trace.csv"), "Dispatch_Id")
+    fr = _gather_rows(os.path.join(fetch_dir, "run_counter_collection.csv"), "Dispatch_Id")
+    wr = _gather_rows(os.path.join(write_dir, "run_counter_collection.csv"), "Dispatch_Id")
+    f = _counter(os.path.join(fetch_dir, "run_counter_collection.csv"), "FETCH_SIZE")
+    w = _counter(os.path.join(write_dir, "run_counter_collection.csv"), "WRITE_SIZE")
+    probe = next(k for k in f if "copy_probe" in k)
+    ff, wf = mib * 1024.0 / f[probe][0], mib * 1024.0 / w[probe][0]
+    n = samples - 1
+    dur, hr, hw = [0.0] * P, [0.0] * P, [0.0] * P
+    for s in range(1, samples):
+        for p in range(P):
+            j = s * P + p
+            want = "1" if pos[p][1] == "norm" else "2"           # gather_kernel<S, kDirect = 1 / kWrite = 2, NKS>
+            for rows in (tr, fr, wr):
+                assert rows[j]["Kernel_Name"].split(",")[2].strip() == want, (j, rows[j]["Kernel_Name"])
+            dur[p] += (int(tr[j]["End_Timestamp"]) - int(tr[j]["Start_Timestamp"])) / 1e3 / n
+            hr[p] += float(fr[j]["Counter_Value"]) * ff * 1024 / n
+            hw[p] += float(wr[j]["Counter_Value"]) * wf * 1024 / n
+    T, A, H = sum(dur), sum(x[5] for x in pos), sum(hr) + sum(hw)
+    out = [f"per sample ({n} samples): gather kernels {T / 1e3:.1f} ms; algorithmic {A / 1e9:.1f} GB "
+           f"({A / T / 1e6:.2f} TB/s); HBM {sum(hr) / 1e9:.1f} GB read + {sum(hw) / 1e9:.1f} GB written "
+           f"({H / T / 1e6:.2f} TB/s, {H / A:.3f} x algorithmic)", "",
+           "| step | pass | r | rn | nker | us | algorithmic GB | HBM GB read + written | TB/s | of 8 TB/s |",
+           "|---|---|---|---|---|---|---|---|---|---|"]
+    for p in range(P):
+        k, kind, r, rn, nk, b = pos[p]
+        if dur[p] < 100:
+            continue
+        out.append(f"| {k} | {kind} | {r} | {rn} | {nk} | {dur[p]:.0f} | {b / 1e9:.2f} | {hr[p] / 1e9:.2f} + "
+                   f"{hw[p] / 1e9:.2f} | {b / dur[p] / 1e6:.2f} | {b / dur[p] / 8e6:.2f} |")
+    return "\n".join(out)
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "stats":
         print(stats(sys.argv[2]))
@@ -104,5 +159,7 @@ if __name__ == "__main__":
         print(launches(sys.argv[2], sys.argv[3], int(sys.argv[4]), [float(x) for x in rest] or None, first, stop))
     elif sys.argv[1] == "traffic":
         print(traffic(sys.argv[2], sys.argv[3], int(sys.argv[4])))
+    elif sys.argv[1] == "gather":
+        print(gather(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], int(sys.argv[6]), int(sys.argv[7])))
     else:
         raise SystemExit(__doc__)
