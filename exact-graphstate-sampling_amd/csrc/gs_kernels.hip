@@ -1288,8 +1288,9 @@ __global__ __launch_bounds__(kThreads) void run_apply_kernel(const RStep* __rest
 // the x = 0 terms, S1 the x = 1 terms with sign (-1)^{parity(c & rho)}; the outputs are M[b][0] S0 + M[b][1] S1.
 // Norm pass (SWEEP = kDirect): both branch norms |M[b][0] S0 + M[b][1] S1|^2 summed directly (the reference's chain).
 // Write pass (kWrite): w0 S0 + w1 S1 stored at nw[cp] (contiguous), with w from the chain's decision or, for the
-// amplitude engine, from wext[4k..4k+3] (the job's per-vertex weights).  Threads own consecutive outputs: output
-// bits 0..7 are the thread index, then the block index, then the Gray-ordered tile index.
+// amplitude engine, from wext[4k..4k+3] (the job's per-vertex weights).  NKS = 2^nker inputs per output.  Output
+// bits 0..7 are the thread index, then the block index, then the tile index (groups of 4 tiles per thread, see
+// below; one tile at a time when a block has fewer than 4).
 template <class S, int SWEEP, int NKS>
 __global__ __launch_bounds__(kThreads) void gather_kernel(const TStep* __restrict__ tp, const S* __restrict__ old,
                                                           S* __restrict__ nw, double* __restrict__ part,
